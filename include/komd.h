@@ -1,0 +1,214 @@
+/*
+ * komd.h -- C ABI of the MI355X-native KoMD force hot path (libkomd_hip.so).
+ *
+ * The library owns device-resident state (positions, velocities, forces, the
+ * linked-cell grid, neighbour lists, surrogate weights) on one HIP stream.
+ * Host buffers passed in or out are owned by the caller and copied.  Every
+ * call completes before it returns, mirroring the Kokkos::fence that ends
+ * each reference functor (potentials/ForceFunctor.cpp:10-14), except
+ * komd_run, which queues many steps and synchronises once at the end.
+ *
+ * Errors are status codes; komd_last_error() returns the message.  Nothing
+ * throws across this boundary.  One context per host thread; a context is
+ * not thread-safe (the reference orchestrates from one thread too).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * KoMD's src/).  Units are the caller's: the library applies no unit
+ * conversion (the reference's conversions, util/constants.h:12-20, live in
+ * the host layer, komd_amd/host/).
+ */
+#ifndef KOMD_H
+#define KOMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    KOMD_OK = 0,
+    KOMD_ERR_ARG = 1,        /* invalid argument (sizes, pointers, modes)          */
+    KOMD_ERR_HIP = 2,        /* HIP runtime error (no device, launch failure, ...)  */
+    KOMD_ERR_STATE = 3,      /* call out of order (e.g. forces before set_sites)    */
+    KOMD_ERR_OVERFLOW = 4,   /* a fixed-capacity buffer overflowed (neighbour list) */
+    KOMD_ERR_NONFINITE = 5,  /* a position/velocity/force became NaN or Inf          */
+    KOMD_ERR_NOMEM = 6       /* device allocation failed                            */
+} komd_status;
+
+/* Binning rule of the container. */
+enum {
+    KOMD_BIN_COM = 0,   /* every site goes to its molecule's COM cell and wraps with
+                           it (LinkedCells.cpp:87-103, :390-400) -- reference        */
+    KOMD_BIN_SITE = 1   /* every site binned and wrapped on its own position       */
+};
+
+/* Three-body force kinds for komd_forces_3b. */
+enum {
+    KOMD_3B_ATM_EXACT = 1,   /* physical ATM, F = -grad V, every triplet within cutoff   */
+    KOMD_3B_ATM_REF = 2,     /* reference ATM_NOLIST semantics: F = +grad V and only
+                                triplets whose cells are all equal or all distinct
+                                (ATM_NOLIST.cpp:20-159, LinkedCells.h:126-152)         */
+    KOMD_3B_SURROGATE = 3,   /* learned descriptor-MLP surrogate of the ATM force         */
+    KOMD_3B_HYBRID = 4       /* exact ATM in a slab, surrogate elsewhere, cos^2 blend    */
+};
+
+/* Pair force kinds for komd_forces_pair. */
+enum { KOMD_PAIR_LJ12_6 = 1 };
+
+typedef struct {
+    double low[3];       /* domain low corner  (config->domainLow)            */
+    double high[3];      /* domain high corner (config->domainHigh)           */
+    double cell_size;    /* linked-cell edge  (config->cell_size), >= cutoff  */
+} komd_box;
+
+typedef struct {
+    double cutoff;        /* LJ and ATM cutoff radius (config->cutoff)                     */
+    double delta_t;       /* time step (config->delta_t)                                   */
+    double limit_factor;  /* Limit clamp fmax = limit*eps/sig (Limit.cpp:19-25); <= 0: off  */
+    double nu;            /* ATM strength, caller units (ATM.cpp:13 converts before use)   */
+    double temp_factor;   /* T = temp_factor * sum(m v.v)/(3 N_in_box) (TemperatureSensor)  */
+    int32_t bin_mode;     /* KOMD_BIN_COM or KOMD_BIN_SITE                                 */
+    int32_t max_neighbors;/* half neighbour list capacity per site (0: 64, max 128)         */
+    int32_t reserved[4];
+} komd_params;
+
+typedef struct {
+    double kinetic;        /* sum 0.5 m v.v over all sites                      */
+    double temperature;    /* temp_factor * sum(m v.v)/(3 N_in_box)             */
+    double u_lj;           /* LJ 12-6 energy, unshifted (LJ12_6_Sensor.cpp:76)  */
+    double u_3b;           /* three-body energy (exact ATM or surrogate)        */
+    double u_bond;         /* harmonic bond energy                              */
+    double u_angle;        /* harmonic angle energy                             */
+    double virial;         /* sum over pairs and triplets of r_i . F_i          */
+    double fmax;           /* largest |F| component                             */
+    int64_t n_in_box;      /* sites inside [low, high] (temperature count)      */
+    int64_t pairs;         /* unique LJ pairs within cutoff at the last pass    */
+    int64_t triplets;      /* unique ATM triplets within cutoff at the last pass */
+} komd_observables;
+
+typedef struct komd_ctx komd_ctx;
+
+/* Context owning device buffers and the HIP stream.  Replaces the Registry-owned
+ * LinkedCells container (main.cpp:48, LinkedCells.cpp:11-45) and force functors
+ * (main.cpp:57-66).  Validates cell_size >= cutoff (FileInput.cpp:117-120) and
+ * at least 3 domain cells per dimension. */
+komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx** out);
+void komd_destroy(komd_ctx* ctx);
+const char* komd_last_error(const komd_ctx* ctx);
+komd_status komd_device_name(char* buf, int32_t len);
+
+/* Domain cells per dimension, floor(L / cell_size) (LinkedCells.cpp:19). */
+komd_status komd_grid_dims(const komd_ctx* ctx, int64_t dims[3]);
+
+/* Copies sites in the reference's SoA/AoS layout (container/SOA.h:11-38):
+ * r, v interleaved d3 (3n doubles), per-site mass/eps/sig, molecule id per site.
+ * Consecutive sites with equal ids form one molecule (CheckpointIO.cpp:63-72).
+ * Replaces MoleculeContainer::constructSOAs (MoleculeContainer.cpp:37-53).
+ * Forces are zeroed. */
+komd_status komd_set_sites(komd_ctx* ctx, int64_t n, const double* r, const double* v, const double* mass,
+                           const double* eps, const double* sig, const uint64_t* mol_id);
+/* Overwrites positions / velocities / forces (any pointer may be NULL). */
+komd_status komd_set_state(komd_ctx* ctx, const double* r, const double* v, const double* f);
+/* Copies state out (any pointer may be NULL); replaces writeSOA2AOS (MoleculeContainer.cpp:24-35). */
+komd_status komd_get_sites(komd_ctx* ctx, double* r, double* v, double* f);
+
+/* Harmonic bonds (U = kb/2 (r-r0)^2) and angles (U = ka/2 (theta-theta0)^2, middle site
+ * second).  New physics: the reference's only bonded force is FENE (potentials/FENE.cpp:15-43). */
+komd_status komd_set_topology(komd_ctx* ctx, int64_t nbonds, const uint32_t* bonds, double kb, double r0,
+                              int64_t nangles, const uint32_t* angles, double ka, double theta0);
+
+/* COM -> periodic wrap -> force reset -> counting-sort binning.  Replaces
+ * LinkedCells::updateContainer (LinkedCells.cpp:65-81). */
+komd_status komd_update_container(komd_ctx* ctx);
+
+/* Bit-exact view of the grid after komd_update_container:
+ *   site_cell[n]      augmented flat cell index x + (nx+2)(y + (ny+2) z) per site
+ *                     (LinkedCells.cpp:92-100, Vec3D.h:73-78)
+ *   cell_count[nc]    sites per domain cell, nc = nx*ny*nz, domain flat order
+ *   cell_start[nc+1]  CSR offsets into sorted_idx
+ *   sorted_idx[n]     site indices grouped by cell, ascending inside a cell
+ *                     (the reference appends in site order, Cell.cpp:32-34)
+ * Any pointer may be NULL. */
+komd_status komd_get_cells(komd_ctx* ctx, int64_t* site_cell, uint32_t* cell_count, uint32_t* cell_start,
+                           uint32_t* sorted_idx);
+
+/* LJ 12-6 over all site pairs of different molecules within cutoff whose cells are
+ * neighbours (ForceFunctor::operator() -> LJ12_6::handlePairList, LJ12_6.cpp:10-43).
+ * Adds to the forces.  Also builds the half neighbour list the 3-body kernels read. */
+komd_status komd_forces_pair(komd_ctx* ctx, int32_t kind);
+/* Harmonic bond and angle forces (new; replaces the reference's FENE functor slot). */
+komd_status komd_forces_bonded(komd_ctx* ctx);
+/* Per-component clamp (Limit::handlePairList, Limit.cpp:19-25); no-op when limit_factor <= 0. */
+komd_status komd_forces_limit(komd_ctx* ctx);
+/* Three-body forces (ForceFunctor3B::operator() -> ATM_NOLIST / ATM, ForceFunctor3B.cpp:10-14). */
+komd_status komd_forces_3b(komd_ctx* ctx, int32_t kind);
+
+/* Split velocity Verlet (Integrator.cpp:10-37). */
+komd_status komd_integrate0(komd_ctx* ctx);
+komd_status komd_integrate1(komd_ctx* ctx);
+/* TemperatureSensor::measure (TemperatureSensor.cpp:22-43). */
+komd_status komd_measure_temperature(komd_ctx* ctx, double* temperature);
+/* VelocityScaling::apply (VelocityScaling.cpp:12-25): v *= sqrt(T0 / T_last_measured). */
+komd_status komd_rescale(komd_ctx* ctx, double t0);
+
+/* Energies, virial and counts of the current configuration (FP64 reductions).
+ * Recomputes the force passes in energy mode; forces are left as they were.
+ * kind3b: 0 none, else a KOMD_3B_* kind.  Replaces LJ12_6_Sensor / PressureSensor. */
+komd_status komd_measure_observables(komd_ctx* ctx, int32_t kind3b, komd_observables* out);
+
+/* Simulation::run main loop (Simulation.cpp:49-74) on the device: nsteps x
+ * {integrate0, update_container, LJ, [bonded], [limit], [3-body], integrate1,
+ *  [temperature + rescale to t0 when t0 > 0]}.  kind3b: 0 none, else KOMD_3B_*.
+ * use_bonded: run the bonded kernel.  Forces must be current on entry
+ * (komd_compute_forces).  Synchronises once at the end. */
+komd_status komd_run(komd_ctx* ctx, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0);
+/* The force part of one step: update_container + LJ + [bonded] + [limit] + [3-body]
+ * (the pre-loop pass of Simulation.cpp:15-24). */
+komd_status komd_compute_forces(komd_ctx* ctx, int32_t kind3b, int32_t use_bonded);
+
+/* Per-kernel device time measured with HIP events on the context's stream.
+ * enable = 1 starts accumulating (komd_run then launches without a graph). */
+komd_status komd_timing_enable(komd_ctx* ctx, int32_t enable);
+/* name: "pair", "atm", "bin_key", "bin_scan", "bin_scatter", "bin_sort", "kick_drift",
+ * "kick", "bonded", "limit", "desc", "mlp", "desc_bwd", ... Total ms and launches. */
+komd_status komd_timing_get(komd_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+/* Cumulative in-range counts since the last reset (for roofline accounting). */
+komd_status komd_counters(komd_ctx* ctx, int64_t* pairs, int64_t* triplets, int32_t reset);
+
+/* ---- learned three-body surrogate ------------------------------------- */
+/* Descriptor-MLP: 32 radial + 16 angular descriptors within cutoff -> 64 -> 64 -> 1,
+ * SiLU, BF16 weights with FP32 accumulation, per-site energy summed; forces = -dE/dr
+ * by a hand-written backward.  No reference counterpart. */
+#define KOMD_SUR_NDESC 48
+#define KOMD_SUR_H1 64
+#define KOMD_SUR_H2 64
+/* Number of float parameters: W1[64x48], b1[64], W2[64x64], b2[64], w3[64], b3[1],
+ * then the 48 descriptor shifts and 48 descriptor scales (input standardisation). */
+#define KOMD_SUR_NPARAM (64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48)
+komd_status komd_surrogate_set_params(komd_ctx* ctx, const float* params, int64_t n);
+komd_status komd_surrogate_get_params(komd_ctx* ctx, float* params, int64_t n);
+/* Descriptors of every site of the current configuration ([n][48], FP32). */
+komd_status komd_surrogate_descriptors(komd_ctx* ctx, float* desc_out);
+/* Per-site surrogate energies ([n], FP32) of the current configuration. */
+komd_status komd_surrogate_energies(komd_ctx* ctx, float* e_out);
+/* Exact per-site ATM energy shares V/3 ([n], FP64) of the current configuration. */
+komd_status komd_atm_site_energies(komd_ctx* ctx, double* e_out);
+/* Training set on the device: append the current configuration's descriptors and
+ * exact per-site ATM energies; clear it. */
+komd_status komd_train_add_snapshot(komd_ctx* ctx);
+komd_status komd_train_clear(komd_ctx* ctx);
+komd_status komd_train_size(komd_ctx* ctx, int64_t* n_samples);
+/* Sets descriptor standardisation from the stored samples (mean, 1/std). */
+komd_status komd_train_standardize(komd_ctx* ctx);
+/* One epoch of minibatch Adam on per-site energies with the project's own BF16
+ * MFMA forward/backward kernels.  Returns the epoch's mean squared error. */
+komd_status komd_train_epoch(komd_ctx* ctx, int32_t batch, float lr, uint64_t seed, double* mse);
+/* Hybrid region: exact ATM for |x - center| < half_width along axis, surrogate
+ * beyond half_width + blend, cos^2 weight in between (AdResS.cpp:60-74 shape). */
+komd_status komd_set_hybrid(komd_ctx* ctx, int32_t axis, double center, double half_width, double blend);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KOMD_H */

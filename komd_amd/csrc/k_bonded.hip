@@ -1,0 +1,134 @@
+// Harmonic bond and angle forces, force clamp -- gfx950.
+#include "komd_internal.hpp"
+
+namespace komd {
+namespace {
+
+// ---- harmonic bonds and angles ---------------------------------------
+// FP64 minimum-image displacements rounded to FP32, FP32 force math, FP64
+// atomic accumulation.  U_bond = kb/2 (r - r0)^2, U_angle = ka/2 (theta - theta0)^2.
+__device__ __forceinline__ float min_image(double d, double L) { return (float)(d - L * rint(d / L)); }
+
+__global__ __launch_bounds__(256) void k_bonds(int nb, const uint32_t* __restrict__ bonds, float kb, float r0,
+                                               const double* __restrict__ rx, const double* __restrict__ ry,
+                                               const double* __restrict__ rz, double* __restrict__ fx,
+                                               double* __restrict__ fy, double* __restrict__ fz, Grid g,
+                                               double* __restrict__ part, int energy) {
+    double u = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+        const uint32_t i = bonds[2 * t], j = bonds[2 * t + 1];
+        const float dx = min_image(rx[j] - rx[i], g.size[0]);
+        const float dy = min_image(ry[j] - ry[i], g.size[1]);
+        const float dz = min_image(rz[j] - rz[i], g.size[2]);
+        const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float dl = len - r0;
+        const float gfac = -kb * dl / len;  // F_j = g d, F_i = -g d
+        unsafeAtomicAdd(&fx[j], (double)(gfac * dx));
+        unsafeAtomicAdd(&fy[j], (double)(gfac * dy));
+        unsafeAtomicAdd(&fz[j], (double)(gfac * dz));
+        unsafeAtomicAdd(&fx[i], (double)(-gfac * dx));
+        unsafeAtomicAdd(&fy[i], (double)(-gfac * dy));
+        unsafeAtomicAdd(&fz[i], (double)(-gfac * dz));
+        if (energy) u += 0.5 * (double)(kb * dl * dl);
+    }
+    if (energy) {
+        __shared__ double sh[256];
+        sh[threadIdx.x] = u;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_angles(int na, const uint32_t* __restrict__ ang, float ka, float th0,
+                                                const double* __restrict__ rx, const double* __restrict__ ry,
+                                                const double* __restrict__ rz, double* __restrict__ fx,
+                                                double* __restrict__ fy, double* __restrict__ fz, Grid g,
+                                                double* __restrict__ part, int energy) {
+    double u = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na; t += gridDim.x * blockDim.x) {
+        const uint32_t i = ang[3 * t], j = ang[3 * t + 1], k = ang[3 * t + 2];
+        const float ax = min_image(rx[i] - rx[j], g.size[0]);
+        const float ay = min_image(ry[i] - ry[j], g.size[1]);
+        const float az = min_image(rz[i] - rz[j], g.size[2]);
+        const float bx = min_image(rx[k] - rx[j], g.size[0]);
+        const float by = min_image(ry[k] - ry[j], g.size[1]);
+        const float bz = min_image(rz[k] - rz[j], g.size[2]);
+        const float la2 = ax * ax + ay * ay + az * az, lb2 = bx * bx + by * by + bz * bz;
+        const float inv_ab = rsqrtf(la2 * lb2);
+        float c = (ax * bx + ay * by + az * bz) * inv_ab;
+        c = fminf(1.f, fmaxf(-1.f, c));
+        const float th = acosf(c);
+        const float dth = th - th0;
+        const float s = fmaxf(sqrtf(fmaxf(0.f, 1.f - c * c)), 1e-6f);
+        const float pre = ka * dth / s;
+        const float ia2 = 1.f / la2, ib2 = 1.f / lb2;
+        const float fix = pre * (bx * inv_ab - c * ax * ia2), fiy = pre * (by * inv_ab - c * ay * ia2),
+                    fiz = pre * (bz * inv_ab - c * az * ia2);
+        const float fkx = pre * (ax * inv_ab - c * bx * ib2), fky = pre * (ay * inv_ab - c * by * ib2),
+                    fkz = pre * (az * inv_ab - c * bz * ib2);
+        unsafeAtomicAdd(&fx[i], (double)fix); unsafeAtomicAdd(&fy[i], (double)fiy); unsafeAtomicAdd(&fz[i], (double)fiz);
+        unsafeAtomicAdd(&fx[k], (double)fkx); unsafeAtomicAdd(&fy[k], (double)fky); unsafeAtomicAdd(&fz[k], (double)fkz);
+        unsafeAtomicAdd(&fx[j], -(double)(fix + fkx));
+        unsafeAtomicAdd(&fy[j], -(double)(fiy + fky));
+        unsafeAtomicAdd(&fz[j], -(double)(fiz + fkz));
+        if (energy) u += 0.5 * (double)(ka * dth * dth);
+    }
+    if (energy) {
+        __shared__ double sh[256];
+        sh[threadIdx.x] = u;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    }
+}
+
+// Limit_Force (Limit.cpp:19-25): f = max(min(f, fmax), -fmax), fmax = limit eps / sig.
+__global__ __launch_bounds__(256) void k_limit(int n, double lim, const double* __restrict__ eps,
+                                               const double* __restrict__ sig, double* __restrict__ fx,
+                                               double* __restrict__ fy, double* __restrict__ fz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double fm = lim * eps[i] / sig[i];
+    double x = fx[i], y = fy[i], z = fz[i];
+    x = x < fm ? x : fm; x = x > -fm ? x : -fm;
+    y = y < fm ? y : fm; y = y > -fm ? y : -fm;
+    z = z < fm ? z : fm; z = z > -fm ? z : -fm;
+    fx[i] = x; fy[i] = y; fz[i] = z;
+}
+
+}  // namespace
+
+hipError_t launch_bonded(komd_ctx* c, bool energy) {
+    ScopedTimer tm(c, "bonded");
+    auto& d = c->d;
+    const int blocks = 256;
+    if (c->nbonds > 0) {
+        k_bonds<<<blocks, 256, 0, c->stream>>>((int)c->nbonds, d.bonds, (float)c->kb, (float)c->r0, d.rx, d.ry, d.rz,
+                                               d.fx, d.fy, d.fz, c->grid, d.red, energy ? 1 : 0);
+        KOMD_CHECK_HIP(hipGetLastError());
+    }
+    if (c->nangles > 0) {
+        k_angles<<<blocks, 256, 0, c->stream>>>((int)c->nangles, d.angles, (float)c->ka, (float)c->theta0, d.rx, d.ry,
+                                                d.rz, d.fx, d.fy, d.fz, c->grid, d.red + blocks, energy ? 1 : 0);
+        KOMD_CHECK_HIP(hipGetLastError());
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_limit(komd_ctx* c) {
+    if (!(c->prm.limit_factor > 0)) return hipSuccess;
+    ScopedTimer tm(c, "limit");
+    auto& d = c->d;
+    const int n = (int)c->n;
+    k_limit<<<(n + 255) / 256, 256, 0, c->stream>>>(n, c->prm.limit_factor, d.eps, d.sig, d.fx, d.fy, d.fz);
+    return hipGetLastError();
+}
+
+}  // namespace komd

@@ -1,0 +1,665 @@
+// C ABI of libkomd_hip.so: context, state transfer, step orchestration.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <vector>
+
+#include "komd_cell.hpp"
+
+namespace komd {
+hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev);
+hipError_t surrogate_energies(komd_ctx* c, float* e_dev);
+hipError_t surrogate_descriptors(komd_ctx* c, float* desc_dev);
+hipError_t hybrid_forces(komd_ctx* c, bool energy);
+
+void timer_begin(komd_ctx* c, const char* name, hipEvent_t* a, hipEvent_t* b) {
+    (void)name;
+    auto grab = [&]() {
+        hipEvent_t e;
+        if (!c->event_pool.empty()) {
+            e = c->event_pool.back();
+            c->event_pool.pop_back();
+        } else {
+            hipEventCreate(&e);
+        }
+        return e;
+    };
+    *a = grab();
+    *b = grab();
+    hipEventRecord(*a, c->stream);
+}
+
+void timer_end(komd_ctx* c, const char* name, hipEvent_t a, hipEvent_t b) {
+    hipEventRecord(b, c->stream);
+    c->pending.push_back({std::string(name), {a, b}});
+}
+}  // namespace komd
+
+using namespace komd;
+
+namespace {
+
+komd_status fail(komd_ctx* c, komd_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+komd_status hip_fail(komd_ctx* c, hipError_t e, const char* where) {
+    return fail(c, KOMD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCALL(c, expr, where)                         \
+    do {                                                \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) return hip_fail(c, _e, where); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+void free_sites(komd_ctx* c) {
+    auto& d = c->d;
+    void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
+                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.spos, d.sprm, d.sid, d.red,
+                    d.esite};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    DevState keep;
+    keep.scal = d.scal;
+    keep.flags = d.flags;
+    keep.counters = d.counters;
+    keep.bonds = d.bonds;
+    keep.angles = d.angles;
+    d = keep;
+    c->have_sites = false;
+    c->nl_valid = false;
+}
+
+void drop_graph(komd_ctx* c) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_key = -1;
+}
+
+// Collect finished event pairs into the per-kernel totals.
+void drain_timers(komd_ctx* c) {
+    for (auto& pe : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+            auto& t = c->timers[pe.first];
+            t.total_ms += ms;
+            t.launches += 1;
+        }
+        c->event_pool.push_back(pe.second.first);
+        c->event_pool.push_back(pe.second.second);
+    }
+    c->pending.clear();
+}
+
+// Synchronises and turns kernel-raised flags into a status.
+komd_status sync_check(komd_ctx* c, const char* where) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, where);
+    if (c->timing) drain_timers(c);
+    uint32_t flags = 0;
+    e = hipMemcpy(&flags, c->d.flags, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, where);
+    if (flags) {
+        uint32_t zero = 0;
+        hipMemcpy(c->d.flags, &zero, sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (flags & FLAG_NL_OVERFLOW)
+            return fail(c, KOMD_ERR_OVERFLOW,
+                        std::string(where) + ": a site has more than 64 in-range neighbours with a higher index");
+        if (flags & FLAG_CELL_OVERFLOW)
+            return fail(c, KOMD_ERR_OVERFLOW,
+                        std::string(where) + ": a cell neighbourhood holds more than 768 sites or a cell more "
+                                             "than 64 (use a smaller cell_size for the three-body pass)");
+        if (flags & FLAG_NONFINITE)
+            return fail(c, KOMD_ERR_NONFINITE, std::string(where) + ": non-finite position or velocity");
+    }
+    return KOMD_OK;
+}
+
+komd_status need_sites(komd_ctx* c) {
+    if (!c) return KOMD_ERR_ARG;
+    if (!c->have_sites) return fail(c, KOMD_ERR_STATE, "no sites: call komd_set_sites first");
+    return KOMD_OK;
+}
+
+bool is_atm(int kind) { return kind == KOMD_3B_ATM_EXACT || kind == KOMD_3B_ATM_REF; }
+int atm_code(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind == KOMD_3B_ATM_REF ? 2 : 0); }
+
+// Three-body forces on their own (ForceFunctor3B::operator(), ForceFunctor3B.cpp:10-14).
+hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
+    if (is_atm(kind)) return launch_cell(c, false, atm_code(kind), energy, false, nullptr, nullptr, nullptr);
+    if (kind == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, energy, 0);
+    if (kind == KOMD_3B_HYBRID) return hybrid_forces(c, energy);
+    return hipErrorInvalidValue;
+}
+
+// The force part of a step in the reference order (Simulation.cpp:55-56, main.cpp:57-66):
+// LJ, bonded (FENE's slot), Limit, then three-body.  The bonded kernel runs first so
+// that the cell-tile pass sees the complete two-body force when it clamps it; LJ,
+// clamp and ATM share that one pass.
+hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
+    if (bonded) KOMD_CHECK_HIP(launch_bonded(c, false));
+    const int atm = atm_code(kind3b);
+    KOMD_CHECK_HIP(launch_cell(c, true, atm, false, true, nullptr, nullptr, nullptr));
+    if (atm) return hipSuccess;
+    if (kind3b == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, false, 0);
+    if (kind3b == KOMD_3B_HYBRID) return hybrid_forces(c, false);
+    return hipSuccess;
+}
+
+// One reference-order step (Simulation.cpp:49-74) queued on the stream.
+hipError_t enqueue_step(komd_ctx* c, int kind3b, bool bonded, bool thermo) {
+    if (thermo)
+        KOMD_CHECK_HIP(launch_com_wrap_key_beta(c, c->d.scal + 2));
+    else
+        KOMD_CHECK_HIP(launch_com_wrap_key(c, true));
+    KOMD_CHECK_HIP(launch_bin(c));
+    KOMD_CHECK_HIP(enqueue_forces(c, kind3b, bonded));
+    KOMD_CHECK_HIP(launch_kick(c, thermo));
+    if (thermo) KOMD_CHECK_HIP(launch_beta_only(c));  // beta for the next step's rescale
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+komd_status komd_device_name(char* buf, int32_t len) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return KOMD_ERR_HIP;
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return KOMD_OK;
+}
+
+komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx** out) {
+    if (!box || !params || !out) return KOMD_ERR_ARG;
+    *out = nullptr;
+    komd_ctx* c = new komd_ctx();
+    c->prm = *params;
+    Grid& g = c->grid;
+    for (int k = 0; k < 3; k++) {
+        g.low[k] = box->low[k];
+        g.high[k] = box->high[k];
+        g.size[k] = box->high[k] - box->low[k];
+    }
+    g.cs = box->cell_size;
+    if (!(params->cutoff > 0) || !(box->cell_size >= params->cutoff)) {
+        // FileInput.cpp:117-120
+        c->err = "cell size must be larger or equal to cutoff radius";
+        *out = c;
+        return KOMD_ERR_ARG;
+    }
+    // LinkedCells.cpp:19: num_cells_domain = floor(dom_size / cell_size)
+    g.nx = (int)floor(g.size[0] / g.cs);
+    g.ny = (int)floor(g.size[1] / g.cs);
+    g.nz = (int)floor(g.size[2] / g.cs);
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3) {
+        c->err = "need at least 3 domain cells per dimension";
+        *out = c;
+        return KOMD_ERR_ARG;
+    }
+    g.ncell = g.nx * g.ny * g.nz;
+    if (c->prm.temp_factor <= 0) c->prm.temp_factor = 1.0;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = dalloc(&c->d.scal, 16);
+    if (e == hipSuccess) e = dalloc(&c->d.flags, 1);
+    if (e == hipSuccess) e = dalloc(&c->d.counters, 2);
+    if (e == hipSuccess) e = hipMemset(c->d.flags, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(c->d.counters, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->d.scal, 0, 16 * sizeof(double));
+    *out = c;
+    if (e != hipSuccess) return hip_fail(c, e, "komd_create");
+    return KOMD_OK;
+}
+
+void komd_destroy(komd_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    surrogate_free(c);
+    free_sites(c);
+    if (c->d.scal) hipFree(c->d.scal);
+    if (c->d.flags) hipFree(c->d.flags);
+    if (c->d.counters) hipFree(c->d.counters);
+    if (c->d.bonds) hipFree(c->d.bonds);
+    if (c->d.angles) hipFree(c->d.angles);
+    for (auto& pe : c->pending) {
+        hipEventDestroy(pe.second.first);
+        hipEventDestroy(pe.second.second);
+    }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* komd_last_error(const komd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+komd_status komd_grid_dims(const komd_ctx* c, int64_t dims[3]) {
+    if (!c || !dims) return KOMD_ERR_ARG;
+    dims[0] = c->grid.nx;
+    dims[1] = c->grid.ny;
+    dims[2] = c->grid.nz;
+    return KOMD_OK;
+}
+
+komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double* v, const double* mass,
+                           const double* eps, const double* sig, const uint64_t* mol_id) {
+    if (!c || n <= 0 || !r || !v || !mass || !eps || !sig || !mol_id) return KOMD_ERR_ARG;
+    if (n > (int64_t)kIdxMask) return fail(c, KOMD_ERR_ARG, "at most 2^26-1 sites");
+    drop_graph(c);
+    free_sites(c);
+    c->n = n;
+    auto& d = c->d;
+    // molecules: runs of equal consecutive ids (CheckpointIO.cpp:63-72); id classes for the exclusion
+    std::vector<uint32_t> mstart;
+    mstart.push_back(0);
+    for (int64_t i = 1; i < n; i++)
+        if (mol_id[i] != mol_id[i - 1]) mstart.push_back((uint32_t)i);
+    mstart.push_back((uint32_t)n);
+    c->nmol = (int64_t)mstart.size() - 1;
+    c->multisite = c->nmol != n;
+    std::map<uint64_t, uint32_t> cls;
+    std::vector<uint32_t> idc((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        auto it = cls.find(mol_id[i]);
+        if (it == cls.end()) it = cls.emplace(mol_id[i], (uint32_t)cls.size()).first;
+        idc[(size_t)i] = it->second;
+    }
+    if (!c->multisite && (int64_t)cls.size() != n) c->multisite = true;  // repeated ids: exclusion matters
+    c->uniform = true;
+    for (int64_t i = 1; i < n; i++)
+        if (eps[i] != eps[0] || sig[i] != sig[0]) { c->uniform = false; break; }
+    c->eps_u = (float)eps[0];
+    c->sig_u = (float)sig[0];
+    // SoA split of the reference's interleaved d3 layout
+    std::vector<double> buf((size_t)n);
+    const size_t nb = (size_t)n * sizeof(double);
+    const size_t C = (size_t)c->grid.ncell;
+    const int pair_blocks = cell_blocks(c);
+    c->red_blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    const size_t red_n = std::max<size_t>({(size_t)3 * pair_blocks, (size_t)3 * c->red_blocks, (size_t)1024}) + 64;
+    hipError_t e = hipSuccess;
+    double** f64[] = {&d.rx, &d.ry, &d.rz, &d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.eps, &d.sig, &d.esite};
+    for (auto pp : f64)
+        if (e == hipSuccess) e = dalloc(pp, (size_t)n);
+    uint32_t** u32n[] = {&d.idcls, &d.key, &d.slot, &d.tmp, &d.sorted, &d.sid};
+    for (auto pp : u32n)
+        if (e == hipSuccess) e = dalloc(pp, (size_t)n);
+    if (e == hipSuccess) e = dalloc(&d.mol_start, mstart.size());
+    if (e == hipSuccess) e = dalloc(&d.count, C);
+    if (e == hipSuccess) e = dalloc(&d.start, C + 1);
+    if (e == hipSuccess) e = dalloc(&d.spos, (size_t)n);
+    if (e == hipSuccess) e = dalloc(&d.sprm, (size_t)n);
+    if (e == hipSuccess) e = dalloc(&d.red, red_n);
+    if (e != hipSuccess) {
+        free_sites(c);
+        return fail(c, KOMD_ERR_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
+    }
+    for (int k = 0; k < 3 && e == hipSuccess; k++) {
+        for (int64_t i = 0; i < n; i++) buf[(size_t)i] = r[3 * i + k];
+        e = hipMemcpy(k == 0 ? d.rx : (k == 1 ? d.ry : d.rz), buf.data(), nb, hipMemcpyHostToDevice);
+        for (int64_t i = 0; i < n && e == hipSuccess; i++) buf[(size_t)i] = v[3 * i + k];
+        if (e == hipSuccess)
+            e = hipMemcpy(k == 0 ? d.vx : (k == 1 ? d.vy : d.vz), buf.data(), nb, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(d.mass, mass, nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.eps, eps, nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.sig, sig, nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.idcls, idc.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.mol_start, mstart.data(), mstart.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d.fx, 0, nb);
+    if (e == hipSuccess) e = hipMemset(d.fy, 0, nb);
+    if (e == hipSuccess) e = hipMemset(d.fz, 0, nb);
+    if (e == hipSuccess) e = hipMemset(d.count, 0, C * 4);
+    if (e != hipSuccess) {
+        free_sites(c);
+        return hip_fail(c, e, "komd_set_sites");
+    }
+    c->have_sites = true;
+    return KOMD_OK;
+}
+
+komd_status komd_set_state(komd_ctx* c, const double* r, const double* v, const double* f) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    auto& d = c->d;
+    const int64_t n = c->n;
+    std::vector<double> buf((size_t)n);
+    const double* src[3] = {r, v, f};
+    double* dst[3][3] = {{d.rx, d.ry, d.rz}, {d.vx, d.vy, d.vz}, {d.fx, d.fy, d.fz}};
+    for (int q = 0; q < 3; q++) {
+        if (!src[q]) continue;
+        for (int k = 0; k < 3; k++) {
+            for (int64_t i = 0; i < n; i++) buf[(size_t)i] = src[q][3 * i + k];
+            HIPCALL(c, hipMemcpy(dst[q][k], buf.data(), (size_t)n * 8, hipMemcpyHostToDevice), "komd_set_state");
+        }
+    }
+    c->nl_valid = false;
+    return KOMD_OK;
+}
+
+komd_status komd_get_sites(komd_ctx* c, double* r, double* v, double* f) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, hipStreamSynchronize(c->stream), "komd_get_sites");
+    auto& d = c->d;
+    const int64_t n = c->n;
+    std::vector<double> buf((size_t)n);
+    double* dst[3] = {r, v, f};
+    double* src[3][3] = {{d.rx, d.ry, d.rz}, {d.vx, d.vy, d.vz}, {d.fx, d.fy, d.fz}};
+    for (int q = 0; q < 3; q++) {
+        if (!dst[q]) continue;
+        for (int k = 0; k < 3; k++) {
+            HIPCALL(c, hipMemcpy(buf.data(), src[q][k], (size_t)n * 8, hipMemcpyDeviceToHost), "komd_get_sites");
+            for (int64_t i = 0; i < n; i++) dst[q][3 * i + k] = buf[(size_t)i];
+        }
+    }
+    return KOMD_OK;
+}
+
+komd_status komd_set_topology(komd_ctx* c, int64_t nb, const uint32_t* bonds, double kb, double r0, int64_t na,
+                              const uint32_t* angles, double ka, double theta0) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (nb < 0 || na < 0 || (nb && !bonds) || (na && !angles)) return fail(c, KOMD_ERR_ARG, "bad topology");
+    for (int64_t i = 0; i < 2 * nb; i++)
+        if ((int64_t)bonds[i] >= c->n) return fail(c, KOMD_ERR_ARG, "bond index out of range");
+    for (int64_t i = 0; i < 3 * na; i++)
+        if ((int64_t)angles[i] >= c->n) return fail(c, KOMD_ERR_ARG, "angle index out of range");
+    drop_graph(c);
+    if (c->d.bonds) hipFree(c->d.bonds);
+    if (c->d.angles) hipFree(c->d.angles);
+    c->d.bonds = c->d.angles = nullptr;
+    HIPCALL(c, dalloc(&c->d.bonds, (size_t)std::max<int64_t>(1, 2 * nb)), "komd_set_topology");
+    HIPCALL(c, dalloc(&c->d.angles, (size_t)std::max<int64_t>(1, 3 * na)), "komd_set_topology");
+    if (nb) HIPCALL(c, hipMemcpy(c->d.bonds, bonds, (size_t)(2 * nb) * 4, hipMemcpyHostToDevice), "komd_set_topology");
+    if (na) HIPCALL(c, hipMemcpy(c->d.angles, angles, (size_t)(3 * na) * 4, hipMemcpyHostToDevice), "komd_set_topology");
+    c->nbonds = nb;
+    c->nangles = na;
+    c->kb = kb;
+    c->r0 = r0;
+    c->ka = ka;
+    c->theta0 = theta0;
+    return KOMD_OK;
+}
+
+komd_status komd_update_container(komd_ctx* c) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_com_wrap_key(c, false), "komd_update_container");
+    HIPCALL(c, launch_bin(c), "komd_update_container");
+    c->nl_valid = false;
+    return sync_check(c, "komd_update_container");
+}
+
+komd_status komd_get_cells(komd_ctx* c, int64_t* site_cell, uint32_t* cell_count, uint32_t* cell_start,
+                           uint32_t* sorted_idx) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, hipStreamSynchronize(c->stream), "komd_get_cells");
+    const Grid& g = c->grid;
+    const int64_t n = c->n;
+    std::vector<uint32_t> start((size_t)g.ncell + 1);
+    HIPCALL(c, hipMemcpy(start.data(), c->d.start, start.size() * 4, hipMemcpyDeviceToHost), "komd_get_cells");
+    if (cell_start) memcpy(cell_start, start.data(), start.size() * 4);
+    if (cell_count)
+        for (int k = 0; k < g.ncell; k++) cell_count[k] = start[(size_t)k + 1] - start[(size_t)k];
+    if (sorted_idx)
+        HIPCALL(c, hipMemcpy(sorted_idx, c->d.sorted, (size_t)n * 4, hipMemcpyDeviceToHost), "komd_get_cells");
+    if (site_cell) {
+        std::vector<uint32_t> key((size_t)n);
+        HIPCALL(c, hipMemcpy(key.data(), c->d.key, (size_t)n * 4, hipMemcpyDeviceToHost), "komd_get_cells");
+        const int64_t ax = g.nx + 2, ay = g.ny + 2;
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t k = key[(size_t)i];
+            const int64_t x = k % g.nx, y = (k / g.nx) % g.ny, z = k / ((int64_t)g.nx * g.ny);
+            site_cell[i] = (x + 1) + ax * ((y + 1) + ay * (z + 1));
+        }
+    }
+    return KOMD_OK;
+}
+
+komd_status komd_forces_pair(komd_ctx* c, int32_t kind) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (kind != KOMD_PAIR_LJ12_6) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
+    HIPCALL(c, launch_cell(c, true, 0, false, false, nullptr, nullptr, nullptr), "komd_forces_pair");
+    return sync_check(c, "komd_forces_pair");
+}
+
+komd_status komd_forces_bonded(komd_ctx* c) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (c->nbonds + c->nangles == 0) return KOMD_OK;
+    HIPCALL(c, launch_bonded(c, false), "komd_forces_bonded");
+    return sync_check(c, "komd_forces_bonded");
+}
+
+komd_status komd_forces_limit(komd_ctx* c) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_limit(c), "komd_forces_limit");
+    return sync_check(c, "komd_forces_limit");
+}
+
+komd_status komd_forces_3b(komd_ctx* c, int32_t kind) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (kind < KOMD_3B_ATM_EXACT || kind > KOMD_3B_HYBRID) return fail(c, KOMD_ERR_ARG, "unknown 3-body kind");
+    hipError_t e = forces_3b(c, kind, false);
+    if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+    HIPCALL(c, e, "komd_forces_3b");
+    return sync_check(c, "komd_forces_3b");
+}
+
+komd_status komd_integrate0(komd_ctx* c) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_kick_drift(c), "komd_integrate0");
+    c->nl_valid = false;
+    return sync_check(c, "komd_integrate0");
+}
+
+komd_status komd_integrate1(komd_ctx* c) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_kick(c, false), "komd_integrate1");
+    return sync_check(c, "komd_integrate1");
+}
+
+komd_status komd_measure_temperature(komd_ctx* c, double* t) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_temperature(c), "komd_measure_temperature");
+    s = sync_check(c, "komd_measure_temperature");
+    if (s) return s;
+    double sc[5];
+    HIPCALL(c, hipMemcpy(sc, c->d.scal, sizeof(sc), hipMemcpyDeviceToHost), "komd_measure_temperature");
+    if (t) *t = sc[3];
+    return KOMD_OK;
+}
+
+komd_status komd_rescale(komd_ctx* c, double t0) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_rescale(c, t0), "komd_rescale");
+    return sync_check(c, "komd_rescale");
+}
+
+komd_status komd_compute_forces(komd_ctx* c, int32_t kind3b, int32_t use_bonded) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    HIPCALL(c, launch_com_wrap_key(c, false), "komd_compute_forces");
+    HIPCALL(c, launch_bin(c), "komd_compute_forces");
+    hipError_t e = enqueue_forces(c, kind3b, use_bonded && (c->nbonds + c->nangles) > 0);
+    if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+    HIPCALL(c, e, "komd_compute_forces");
+    return sync_check(c, "komd_compute_forces");
+}
+
+komd_status komd_run(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (nsteps < 0) return fail(c, KOMD_ERR_ARG, "negative step count");
+    if (kind3b < 0 || kind3b > KOMD_3B_HYBRID) return fail(c, KOMD_ERR_ARG, "unknown 3-body kind");
+    const bool bonded = use_bonded && (c->nbonds + c->nangles) > 0;
+    const bool thermo = t0 > 0;
+    if (thermo) {
+        const double one = 1.0;
+        HIPCALL(c, hipMemcpyAsync(c->d.scal + 2, &one, sizeof(double), hipMemcpyHostToDevice, c->stream), "komd_run");
+        HIPCALL(c, hipStreamSynchronize(c->stream), "komd_run");
+    }
+    c->run_t0 = t0;
+    if (c->timing || nsteps < 3) {
+        for (int64_t k = 0; k < nsteps; k++) {
+            hipError_t e = enqueue_step(c, kind3b, bonded, thermo);
+            if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+            HIPCALL(c, e, "komd_run");
+            if (c->timing && (k % 64) == 63) {
+                s = sync_check(c, "komd_run");
+                if (s) return s;
+            }
+        }
+    } else {
+        const int key = kind3b * 4 + (bonded ? 2 : 0) + (thermo ? 1 : 0);
+        if (!c->graph_exec || c->graph_key != key) {
+            drop_graph(c);
+            hipGraph_t graph = nullptr;
+            HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "komd_run capture");
+            hipError_t e = enqueue_step(c, kind3b, bonded, thermo);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if (e == hipErrorNotSupported) {
+                if (graph) hipGraphDestroy(graph);
+                return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+            }
+            HIPCALL(c, e, "komd_run capture");
+            HIPCALL(c, e2, "komd_run capture");
+            e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            HIPCALL(c, e, "komd_run instantiate");
+            c->graph_key = key;
+        }
+        for (int64_t k = 0; k < nsteps; k++) {
+            HIPCALL(c, hipGraphLaunch(c->graph_exec, c->stream), "komd_run launch");
+            if ((k % 256) == 255) {
+                s = sync_check(c, "komd_run");
+                if (s) return s;
+            }
+        }
+    }
+    if (thermo && nsteps > 0) HIPCALL(c, launch_rescale(c, t0), "komd_run");
+    return sync_check(c, "komd_run");
+}
+
+komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* out) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (!out) return KOMD_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    auto& d = c->d;
+    const int64_t n = c->n;
+    // keep the current forces: the energy passes rebuild them
+    std::vector<double> fsave((size_t)(3 * n));
+    s = komd_get_sites(c, nullptr, nullptr, fsave.data());
+    if (s) return s;
+    unsigned long long cnt0[2];
+    HIPCALL(c, hipMemcpy(cnt0, d.counters, sizeof(cnt0), hipMemcpyDeviceToHost), "komd_measure_observables");
+    HIPCALL(c, launch_com_wrap_key(c, false), "komd_measure_observables");
+    HIPCALL(c, launch_bin(c), "komd_measure_observables");
+    double* out_dev = d.scal + 8;
+    double vals[8];
+    const int atm = atm_code(kind3b);
+    HIPCALL(c, launch_cell(c, true, atm, true, false, nullptr, nullptr, nullptr), "komd_measure_observables");
+    const int pb = cell_blocks(c);
+    HIPCALL(c, launch_reduce_parts(c, d.red, pb, 3, out_dev), "komd_measure_observables");
+    s = sync_check(c, "komd_measure_observables");
+    if (s) return s;
+    HIPCALL(c, hipMemcpy(vals, out_dev, 3 * sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+    out->u_lj = vals[0];
+    out->virial = vals[1];
+    out->u_3b = vals[2];
+    if (c->nbonds + c->nangles > 0) {
+        HIPCALL(c, hipMemset(d.red, 0, 512 * sizeof(double)), "komd_measure_observables");
+        HIPCALL(c, launch_bonded(c, true), "komd_measure_observables");
+        HIPCALL(c, launch_reduce_parts(c, d.red, 256, 2, out_dev), "komd_measure_observables");
+        s = sync_check(c, "komd_measure_observables");
+        if (s) return s;
+        HIPCALL(c, hipMemcpy(vals, out_dev, 2 * sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+        out->u_bond = vals[0];
+        out->u_angle = vals[1];
+    }
+    if (kind3b && !atm) {
+        hipError_t e = forces_3b(c, kind3b, true);
+        if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+        HIPCALL(c, e, "komd_measure_observables");
+        HIPCALL(c, launch_reduce_parts(c, d.red, c->red_blocks, 1, out_dev), "komd_measure_observables");
+        s = sync_check(c, "komd_measure_observables");
+        if (s) return s;
+        HIPCALL(c, hipMemcpy(vals, out_dev, sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+        out->u_3b = vals[0];
+    }
+    unsigned long long cnt1[2];
+    HIPCALL(c, hipMemcpy(cnt1, d.counters, sizeof(cnt1), hipMemcpyDeviceToHost), "komd_measure_observables");
+    out->pairs = (int64_t)(cnt1[0] - cnt0[0]);
+    out->triplets = (int64_t)(cnt1[1] - cnt0[1]) / 3;  // every triplet is visited by its 3 members
+    // kinetic energy and temperature
+    HIPCALL(c, launch_temperature(c), "komd_measure_observables");
+    s = sync_check(c, "komd_measure_observables");
+    if (s) return s;
+    double sc[5];
+    HIPCALL(c, hipMemcpy(sc, d.scal, sizeof(sc), hipMemcpyDeviceToHost), "komd_measure_observables");
+    out->temperature = sc[3];
+    out->kinetic = sc[4];
+    out->n_in_box = (int64_t)sc[1];
+    // forces back as they were; fmax from them
+    double fm = 0;
+    for (double x : fsave) fm = std::max(fm, fabs(x));
+    out->fmax = fm;
+    return komd_set_state(c, nullptr, nullptr, fsave.data());
+}
+
+komd_status komd_timing_enable(komd_ctx* c, int32_t enable) {
+    if (!c) return KOMD_ERR_ARG;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drain_timers(c);
+    c->timing = enable != 0;
+    c->timers.clear();
+    return KOMD_OK;
+}
+
+komd_status komd_timing_get(komd_ctx* c, const char* name, double* total_ms, int64_t* launches) {
+    if (!c || !name) return KOMD_ERR_ARG;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drain_timers(c);
+    auto it = c->timers.find(name);
+    if (total_ms) *total_ms = it == c->timers.end() ? 0.0 : it->second.total_ms;
+    if (launches) *launches = it == c->timers.end() ? 0 : it->second.launches;
+    return KOMD_OK;
+}
+
+komd_status komd_counters(komd_ctx* c, int64_t* pairs, int64_t* triplets, int32_t reset) {
+    if (!c) return KOMD_ERR_ARG;
+    unsigned long long v[2] = {0, 0};
+    if (c->d.counters) {
+        HIPCALL(c, hipStreamSynchronize(c->stream), "komd_counters");
+        HIPCALL(c, hipMemcpy(v, c->d.counters, sizeof(v), hipMemcpyDeviceToHost), "komd_counters");
+        if (reset) HIPCALL(c, hipMemset(c->d.counters, 0, sizeof(v)), "komd_counters");
+    }
+    if (pairs) *pairs = (int64_t)v[0];
+    if (triplets) *triplets = (int64_t)v[1] / 3;  // every triplet is visited by its 3 members
+    return KOMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Internal state and launch wrappers of libkomd_hip.so (gfx950).
+//
+// Device layout (HBM), N sites, C = nx*ny*nz domain cells:
+//   original site order, FP64 SoA : rx ry rz vx vy vz fx fy fz (9 x 8N B), mass, eps, sig
+//   idcls[N] u32   dense molecule-id class (LJ same-molecule exclusion, LJ12_6.cpp:19)
+//   mol_of[N] u32, mol_start[M+1] u32   molecule grouping for the COM (MoleculeContainer.cpp:55-75)
+//   key[N] u32     domain flat cell of every site (x + nx (y + ny z), 0-based)
+//   slot[N] u32    arrival slot inside its cell (atomic histogram)
+//   count[C], start[C+1] u32, tmp[N], sorted[N] u32   counting sort, stable per cell
+//   spos[N] float4 cell-sorted FP32 position relative to the cell centre, .w = site index bits
+//   sprm[N] float2 cell-sorted (sigma, sqrt(eps)) when the sites are not uniform
+//   sid[N]  u32    cell-sorted id class when molecules have several sites
+//   esite[N] f64   per-site three-body energy share (training targets)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/komd.h"
+
+namespace komd {
+
+constexpr uint32_t kIdxBits = 26;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+
+// error flags raised by kernels (bitmask in dev->flags)
+enum : uint32_t { FLAG_NL_OVERFLOW = 1u, FLAG_NONFINITE = 2u, FLAG_CELL_OVERFLOW = 4u };
+
+struct Grid {
+    int nx, ny, nz;         // domain cells
+    int ncell;              // nx*ny*nz
+    double low[3], high[3], size[3], cs;
+};
+
+struct Partials {
+    double* buf = nullptr;  // per-block FP64 partial sums, [nslots][nblocks]
+    int nblocks = 0;
+};
+
+struct Surrogate;  // k_surrogate.hip
+
+struct DevState {
+    // sites
+    double *rx = nullptr, *ry = nullptr, *rz = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    double *mass = nullptr, *eps = nullptr, *sig = nullptr;
+    uint32_t* idcls = nullptr;
+    uint32_t* mol_start = nullptr;
+    // binning
+    uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr,
+             *sorted = nullptr;
+    float4* spos = nullptr;
+    float2* sprm = nullptr;
+    uint32_t* sid = nullptr;
+    // bonded
+    uint32_t *bonds = nullptr, *angles = nullptr;
+    // reductions and flags
+    double* red = nullptr;        // partials (see k_container.hip)
+    double* scal = nullptr;       // device scalars: [0] mv2, [1] count, [2] beta, ...
+    uint32_t* flags = nullptr;
+    unsigned long long* counters = nullptr;  // [0] pairs, [1] triplets
+    double* esite = nullptr;      // per-site energy scratch [N]
+};
+
+struct Timer {
+    double total_ms = 0;
+    int64_t launches = 0;
+};
+
+}  // namespace komd
+
+struct komd_ctx {
+    komd::Grid grid{};
+    komd_params prm{};
+    int64_t n = 0;
+    int64_t nmol = 0;
+    bool uniform = true;       // all sites share sigma/epsilon
+    bool multisite = false;    // some molecule has more than one site
+    float sig_u = 1.f, eps_u = 1.f;
+    int64_t nbonds = 0, nangles = 0;
+    double kb = 0, r0 = 0, ka = 0, theta0 = 0;
+    hipStream_t stream = nullptr;
+    komd::DevState d;
+    std::string err;
+    bool have_sites = false;
+    bool nl_valid = false;
+    int red_blocks = 0;
+    // timing
+    bool timing = false;
+    std::unordered_map<std::string, komd::Timer> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    // graph of one step
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key = -1;
+    // surrogate
+    komd::Surrogate* sur = nullptr;
+    // hybrid
+    double run_t0 = 0;         // thermostat target of the running komd_run
+    int hyb_axis = 0;
+    double hyb_center = 0, hyb_half = 0, hyb_blend = 0;
+};
+
+namespace komd {
+
+// kernel timing helpers (komd_api.hip)
+void timer_begin(komd_ctx* c, const char* name, hipEvent_t* a, hipEvent_t* b);
+void timer_end(komd_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
+
+struct ScopedTimer {
+    komd_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ScopedTimer(komd_ctx* c_, const char* n_) : c(c_), name(n_) {
+        if (c->timing) timer_begin(c, name, &a, &b);
+    }
+    ~ScopedTimer() {
+        if (c->timing) timer_end(c, name, a, b);
+    }
+};
+
+// ---- launch wrappers --------------------------------------------------
+// k_container.hip
+hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift);
+hipError_t launch_bin(komd_ctx* c);             // scan + scatter + stable sort + FP32 repack
+hipError_t launch_kick_drift(komd_ctx* c);      // integrate0
+hipError_t launch_kick(komd_ctx* c, bool with_temp);  // integrate1 (+ kinetic partials)
+hipError_t launch_temperature(komd_ctx* c);     // kinetic partials -> scal[0..1]
+hipError_t launch_rescale(komd_ctx* c, double t0);    // beta from scal, v *= beta
+hipError_t launch_beta_only(komd_ctx* c);             // beta = sqrt(run_t0/T) into scal[2]
+hipError_t launch_reduce_parts(komd_ctx* c, const double* part, int nparts, int nslots, double* out_dev);
+// k_cell.hip: LJ, clamp and ATM on cell tiles.  atm: 0 off, 1 physical, 2 reference.
+// limit: clamp the two-body force before the three-body term is added.
+// lam/dlam: hybrid weights (nullptr otherwise); esite: per-site ATM energy out (optional).
+hipError_t launch_cell(komd_ctx* c, bool lj, int atm, bool energy, bool limit, const float* lam, const float* dlam,
+                       double* esite);
+int cell_blocks(komd_ctx* c);
+// k_bonded.hip
+hipError_t launch_bonded(komd_ctx* c, bool energy);
+hipError_t launch_limit(komd_ctx* c);
+// k_surrogate.hip
+hipError_t launch_surrogate_forces(komd_ctx* c, bool energy, int hybrid);
+void surrogate_free(komd_ctx* c);
+
+}  // namespace komd
+
+#define KOMD_CHECK_HIP(expr)                                                          \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) return _e;                                             \
+    } while (0)

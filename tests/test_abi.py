@@ -1,0 +1,24 @@
+"""C ABI: the library loads and exports every entry point include/komd.h declares
+(no compute calls, so this runs without a GPU)."""
+import ctypes
+import os
+
+import komd_amd as K
+
+
+def test_library_exports_every_declared_symbol():
+    lib = K.load_library()
+    names = K.declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_wrapper_signatures_cover_the_header():
+    names = set(K.declared_symbols())
+    assert names == set(K._SIGNATURES), names ^ set(K._SIGNATURES)
+
+
+def test_library_is_in_tree():
+    assert os.path.dirname(K.LIB_PATH) == os.path.dirname(os.path.abspath(K.__file__))
+    ctypes.CDLL(K.LIB_PATH)

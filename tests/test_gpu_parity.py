@@ -1,0 +1,240 @@
+"""GPU parity of the hot path through the C ABI against the FP64 oracle.
+
+Bars (stated per test):
+  * binning: bit-exact (cell of every site, per-cell counts, per-cell order);
+  * integrator: bit-exact (same FP64 operation order, no contraction);
+  * LJ / ATM / bonded forces: FP32 pair/triplet arithmetic on relative
+    coordinates with FP64 accumulation -> max and RMS error relative to the
+    RMS of the FP64 oracle force, tolerances written in each test.
+"""
+import numpy as np
+import pytest
+
+import komd_amd as K
+from komd_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.073
+
+
+def rel_err(a, b):
+    scale = np.sqrt(np.mean(b * b))
+    d = np.linalg.norm(a - b, axis=1)
+    return float(d.max() / scale), float(np.sqrt(np.mean(d * d)) / scale)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle
+    oracle.build()
+    return oracle
+
+
+def small(seed=11, cells=5, jitter=0.12, T=0.8):
+    return S.fcc(cells, 0.8, T, seed, jitter=jitter)
+
+
+@pytest.mark.parametrize("cells,seed", [(5, 11), (10, 12), (20, 13)])
+def test_binning_bit_exact(orc, cells, seed):
+    s = small(seed, cells)
+    # push some sites outside the box so the wrap path runs
+    s.r[::97] += np.array([s.box[0] * 0.999, -0.3, 0.0])
+    s.r[5::131] -= np.array([0.0, 0.0, s.box[2] * 1.0001])
+    print(f"binning cells={cells} seed={seed} n={s.n}", flush=True)
+    r_or, b = orc.update_container(s)
+    with K.Context(s) as c:
+        c.update_container()
+        site_cell, count, start, srt = c.get_cells()
+        r_gpu, _, f = c.get_state()
+    assert np.array_equal(r_gpu, r_or), "wrapped positions differ"
+    assert np.array_equal(site_cell, b.site_cell)
+    nd = b.nd
+    na = nd + 2
+    # domain cells of the oracle's augmented grid, in domain flat order
+    gz, gy, gx = np.meshgrid(np.arange(1, nd[2] + 1), np.arange(1, nd[1] + 1), np.arange(1, nd[0] + 1),
+                             indexing="ij")
+    aug = (gx + na[0] * (gy + na[1] * gz)).ravel()
+    assert np.array_equal(count.astype(np.int64), b.cell_count[aug])
+    for k in range(len(aug)):
+        ref = b.cell_list[b.cell_start[aug[k]]:b.cell_start[aug[k] + 1]]
+        got = srt[start[k]:start[k + 1]]
+        assert np.array_equal(got.astype(np.int64), ref), f"cell {k} order"
+    assert not f.any(), "forces not reset"
+
+
+def test_binning_multisite_com(orc):
+    s = S.chains((6, 6, 6), 4, 0.7, seed=5)
+    s.r[:8] += s.box[0] * 0.51  # move two molecules across the boundary
+    print(f"binning chains n={s.n}", flush=True)
+    r_or, b = orc.update_container(s)
+    with K.Context(s) as c:
+        c.update_container()
+        site_cell, count, start, srt = c.get_cells()
+        r_gpu, _, _ = c.get_state()
+    assert np.array_equal(r_gpu, r_or)
+    assert np.array_equal(site_cell, b.site_cell)
+
+
+def test_integrator_bit_exact(orc):
+    s = small(21, 5)
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=(s.n, 3)) * 10
+    s.mass = 1.0 + rng.random(s.n)
+    with K.Context(s, dt=0.005) as c:
+        c.set_state(f=f)
+        c.integrate0()
+        r1, v1, _ = c.get_state()
+        r_o, v_o = orc.vv_step0(0.005, s.r, s.v, f, s.mass)
+        assert np.array_equal(r1, r_o) and np.array_equal(v1, v_o)
+        c.integrate1()
+        _, v2, _ = c.get_state()
+        assert np.array_equal(v2, orc.vv_step1(0.005, v_o, f, s.mass))
+
+
+def test_temperature_and_rescale(orc):
+    s = small(22, 5, T=1.3)
+    with K.Context(s) as c:
+        T = c.temperature()
+        mv2, cnt = orc.kinetic_sum(s, s.r, s.v)
+        assert abs(T - mv2 / (3 * cnt)) <= 1e-12 * T
+        c.rescale(0.8)
+        _, v, _ = c.get_state()
+        np.testing.assert_allclose(v, s.v * np.sqrt(0.8 / T), rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("cells,seed", [(5, 31), (10, 32), (20, 33)])
+def test_lj_forces(orc, cells, seed):
+    s = small(seed, cells)
+    print(f"lj cells={cells} n={s.n}", flush=True)
+    r, b = orc.update_container(s)
+    f_or, u_or, w_or = orc.lj_c08(s, r, b, energy=True, nthreads=8)
+    with K.Context(s) as c:
+        c.update_container()
+        c.forces_pair()
+        f = c.forces()
+        obs = c.observables()
+    mx, rms = rel_err(f, f_or)
+    print(f"LJ max {mx:.3e} rms {rms:.3e}")
+    # FP32 relative coordinates: close contacts carry ~1e-5 relative force error
+    assert mx < 2e-4 and rms < 5e-6
+    # per-pair energies are FP32 (summed in FP64): relative bar 1e-5
+    assert abs(obs["u_lj"] - u_or) < 1e-5 * abs(u_or)
+
+
+def test_lj_mixing_and_exclusion(orc):
+    """Per-site sigma/epsilon (Lorentz-Berthelot, LJ12_6.cpp:31-32) and same-molecule exclusion."""
+    s = S.chains((5, 5, 5), 4, 0.5, seed=8)
+    rng = np.random.default_rng(4)
+    s.sig = 0.9 + 0.2 * rng.random(s.n)
+    s.eps = 0.5 + rng.random(s.n)
+    r, b = orc.update_container(s)
+    f_or = orc.lj_c08(s, r, b)
+    with K.Context(s) as c:
+        c.update_container()
+        c.forces_pair()
+        f = c.forces()
+    mx, rms = rel_err(f, f_or)
+    print(f"LJ mix max {mx:.3e} rms {rms:.3e}")
+    assert mx < 2e-4 and rms < 5e-6
+
+
+@pytest.mark.parametrize("cells,seed", [(5, 41), (10, 42), (20, 43)])
+def test_atm_exact_forces(orc, cells, seed):
+    s = small(seed, cells)
+    print(f"atm exact cells={cells} n={s.n}", flush=True)
+    r, b = orc.update_container(s)
+    f_or, e_or, hits = orc.atm_complete(s, r, b, NU, coverage=0, sign=-1.0, nthreads=8)
+    with K.Context(s, nu=NU) as c:
+        c.update_container()
+        c.counters(reset=True)
+        c.forces_3b(K.ATM_EXACT)
+        f = c.forces()
+        _, trip = c.counters()
+        obs = c.observables(K.ATM_EXACT)
+    assert trip == hits, "triplet set differs"
+    mx, rms = rel_err(f, f_or)
+    print(f"ATM max {mx:.3e} rms {rms:.3e} triplets {hits}")
+    assert mx < 1e-4 and rms < 1e-5
+    assert abs(obs["u_3b"] - e_or) < 1e-5 * abs(e_or)
+
+
+@pytest.mark.parametrize("cells,seed", [(5, 51), (10, 52)])
+def test_atm_reference_mode(orc, cells, seed):
+    """Reference ATM_NOLIST semantics: +grad V on the C08 triplet set."""
+    s = small(seed, cells)
+    r, b = orc.update_container(s)
+    f_or, _ = orc.atm_c08(s, r, b, NU, sign=1.0, nthreads=8)
+    with K.Context(s, nu=NU) as c:
+        c.update_container()
+        c.forces_3b(K.ATM_REF)
+        f = c.forces()
+    mx, rms = rel_err(f, f_or)
+    print(f"ATM ref max {mx:.3e} rms {rms:.3e}")
+    assert mx < 1e-4 and rms < 1e-5
+
+
+def test_bonded_forces(orc):
+    s = S.chains((6, 6, 6), 4, 0.7, seed=9)
+    # the generator places bonds and angles at their rest values: perturb them
+    s.r = s.r + np.random.default_rng(9).normal(size=s.r.shape) * 0.05
+    r, _ = orc.update_container(s)
+    f_or, u_or = orc.bonded(s, r)
+    with K.Context(s, topology={"bonds": s.bonds, "angles": s.angles}) as c:
+        c.update_container()
+        c.forces_bonded()
+        f = c.forces()
+        obs = c.observables()
+    mx, rms = rel_err(f, f_or)
+    print(f"bonded max {mx:.3e} rms {rms:.3e}")
+    assert mx < 1e-4 and rms < 1e-5
+    assert abs(obs["u_bond"] + obs["u_angle"] - u_or) < 1e-4 * abs(u_or)
+
+
+def test_limit(orc):
+    s = small(61, 5, jitter=0.15)
+    r, b = orc.update_container(s)
+    f_or = orc.limit(s, orc.lj_c08(s, r, b), 20.0)
+    with K.Context(s, limit_factor=20.0) as c:
+        c.update_container()
+        c.forces_pair()
+        c.forces_limit()
+        f = c.forces()
+    assert np.abs(f).max() <= 20.0
+    mx, _ = rel_err(f, f_or)
+    assert mx < 5e-5
+
+
+def test_run_matches_oracle_loop(orc):
+    """10 reference-order steps with the rescale thermostat (Simulation.cpp:49-74)."""
+    s = small(71, 6, jitter=0.05)
+    r0, b = orc.update_container(s)
+    f0 = orc.lj_c08(s, r0, b) + orc.atm_c08(s, r0, b, NU, sign=1.0)[0]
+    r_o, v_o, f_o = orc.run(s, 10, dt=0.005, nu=NU, atm=1, atm_sign=1.0, T0=0.8, temp_factor=1.0, nthreads=8,
+                            f=f0, r=r0)
+    with K.Context(s, nu=NU, dt=0.005) as c:
+        c.compute_forces(K.ATM_REF)
+        c.run(10, kind3b=K.ATM_REF, t0=0.8)
+        r, v, f = c.get_state()
+    # forces carry FP32 rounding, so trajectories agree to a tolerance only
+    assert np.abs(r - r_o).max() < 1e-6
+    assert np.abs(v - v_o).max() < 1e-5
+
+
+def test_graph_replays_match_eager(orc):
+    s = small(81, 6, jitter=0.05)
+    with K.Context(s, nu=NU) as a, K.Context(s, nu=NU) as b:
+        a.compute_forces(K.ATM_EXACT)
+        b.compute_forces(K.ATM_EXACT)
+        for _ in range(3):
+            ra, va, fa = a.get_state()
+            b.set_state(ra, va, fa)                # same start; only summation order differs
+            a.run(8, kind3b=K.ATM_EXACT)          # graph path (nsteps >= 3)
+            b.timing(True)
+            b.run(8, kind3b=K.ATM_EXACT)          # eager path (timing on)
+            b.timing(False)
+            ra, va, _ = a.get_state()
+            rb, vb, _ = b.get_state()
+            # FP32 LDS accumulation order is not fixed: differences stay at rounding level,
+            # a stale or missing graph node would show up as O(1e-3)
+            assert np.abs(ra - rb).max() < 1e-9 and np.abs(va - vb).max() < 2e-7
