@@ -102,10 +102,11 @@ def main():
     # ---- per-kernel device time with HIP events over a second pass of K steps
     ctx.timing(True)
     ctx.run(args.steps, kind3b=K.ATM_EXACT)
-    kms, kl = ctx.timer("cell_lj_atm")
+    kms, kl = ctx.timer("atm")
+    pms, pl = ctx.timer("cell_lj_nl")
     p2, t2 = ctx.counters(reset=True)
     kernels = {}
-    for name in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj_atm", "kick"):
+    for name in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj_nl", "atm", "kick"):
         ms, cnt = ctx.timer(name)
         kernels[name] = round(ms / max(cnt, 1) * 1e3, 2)
     ctx.timing(False)

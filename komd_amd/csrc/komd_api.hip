@@ -67,7 +67,7 @@ hipError_t dalloc(T** p, size_t count) {
 void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
-                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.spos, d.sprm, d.sid, d.red,
+                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.red, d.cnt_pair, d.cnt_trip,
                     d.esite};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -127,6 +127,22 @@ komd_status sync_check(komd_ctx* c, const char* where) {
     return KOMD_OK;
 }
 
+// Sums the per-workgroup count slots: [0] unique in-range pairs, [1] triplet visits.
+komd_status read_counters(komd_ctx* c, int64_t out[2], bool reset) {
+    HIPCALL(c, hipStreamSynchronize(c->stream), "komd_counters");
+    const int nb[2] = {cell_blocks(c), atm_blocks(c)};
+    unsigned long long* src[2] = {c->d.cnt_pair, c->d.cnt_trip};
+    for (int k = 0; k < 2; k++) {
+        std::vector<unsigned long long> h((size_t)nb[k]);
+        HIPCALL(c, hipMemcpy(h.data(), src[k], h.size() * 8, hipMemcpyDeviceToHost), "komd_counters");
+        unsigned long long t = 0;
+        for (auto x : h) t += x;
+        out[k] = (int64_t)t;
+        if (reset) HIPCALL(c, hipMemset(src[k], 0, h.size() * 8), "komd_counters");
+    }
+    return KOMD_OK;
+}
+
 komd_status need_sites(komd_ctx* c) {
     if (!c) return KOMD_ERR_ARG;
     if (!c->have_sites) return fail(c, KOMD_ERR_STATE, "no sites: call komd_set_sites first");
@@ -134,11 +150,24 @@ komd_status need_sites(komd_ctx* c) {
 }
 
 bool is_atm(int kind) { return kind == KOMD_3B_ATM_EXACT || kind == KOMD_3B_ATM_REF; }
+
+// Neighbour-list buffers, allocated on first use by a three-body or surrogate pass.
+hipError_t ensure_nl(komd_ctx* c, bool dirs) {
+    auto& d = c->d;
+    if (!d.nl) KOMD_CHECK_HIP(dalloc(&d.nl, (size_t)c->n * kFullCap));
+    if (!d.nlc) KOMD_CHECK_HIP(dalloc(&d.nlc, (size_t)c->n));
+    if (dirs && !d.nld) KOMD_CHECK_HIP(dalloc(&d.nld, (size_t)c->n * kFullCap));
+    return hipSuccess;
+}
 int atm_code(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind == KOMD_3B_ATM_REF ? 2 : 0); }
 
 // Three-body forces on their own (ForceFunctor3B::operator(), ForceFunctor3B.cpp:10-14).
 hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
-    if (is_atm(kind)) return launch_cell(c, false, atm_code(kind), energy, false, nullptr, nullptr, nullptr);
+    if (is_atm(kind)) {
+        KOMD_CHECK_HIP(ensure_nl(c, kind == KOMD_3B_ATM_REF));
+        KOMD_CHECK_HIP(launch_cell(c, false, atm_code(kind), false, false));
+        return launch_atm(c, atm_code(kind), energy, nullptr, nullptr, nullptr);
+    }
     if (kind == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, energy, 0);
     if (kind == KOMD_3B_HYBRID) return hybrid_forces(c, energy);
     return hipErrorInvalidValue;
@@ -151,8 +180,10 @@ hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
 hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     if (bonded) KOMD_CHECK_HIP(launch_bonded(c, false));
     const int atm = atm_code(kind3b);
-    KOMD_CHECK_HIP(launch_cell(c, true, atm, false, true, nullptr, nullptr, nullptr));
-    if (atm) return hipSuccess;
+    const int nl = atm ? atm : (kind3b ? 1 : 0);
+    if (nl) KOMD_CHECK_HIP(ensure_nl(c, nl == 2));
+    KOMD_CHECK_HIP(launch_cell(c, true, nl, false, true));
+    if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr);
     if (kind3b == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, false, 0);
     if (kind3b == KOMD_3B_HYBRID) return hybrid_forces(c, false);
     return hipSuccess;
@@ -289,7 +320,8 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     const size_t C = (size_t)c->grid.ncell;
     const int pair_blocks = cell_blocks(c);
     c->red_blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
-    const size_t red_n = std::max<size_t>({(size_t)3 * pair_blocks, (size_t)3 * c->red_blocks, (size_t)1024}) + 64;
+    const size_t red_n = std::max<size_t>({(size_t)2 * pair_blocks, (size_t)atm_blocks(c), (size_t)3 * c->red_blocks,
+                                           (size_t)1024}) + 64;
     hipError_t e = hipSuccess;
     double** f64[] = {&d.rx, &d.ry, &d.rz, &d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.eps, &d.sig, &d.esite};
     for (auto pp : f64)
@@ -303,6 +335,10 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     if (e == hipSuccess) e = dalloc(&d.spos, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.sprm, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.red, red_n);
+    if (e == hipSuccess) e = dalloc(&d.cnt_pair, (size_t)cell_blocks(c));
+    if (e == hipSuccess) e = dalloc(&d.cnt_trip, (size_t)atm_blocks(c));
+    if (e == hipSuccess) e = hipMemset(d.cnt_pair, 0, (size_t)cell_blocks(c) * 8);
+    if (e == hipSuccess) e = hipMemset(d.cnt_trip, 0, (size_t)atm_blocks(c) * 8);
     if (e != hipSuccess) {
         free_sites(c);
         return fail(c, KOMD_ERR_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
@@ -435,7 +471,7 @@ komd_status komd_forces_pair(komd_ctx* c, int32_t kind) {
     komd_status s = need_sites(c);
     if (s) return s;
     if (kind != KOMD_PAIR_LJ12_6) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
-    HIPCALL(c, launch_cell(c, true, 0, false, false, nullptr, nullptr, nullptr), "komd_forces_pair");
+    HIPCALL(c, launch_cell(c, true, 0, false, false), "komd_forces_pair");
     return sync_check(c, "komd_forces_pair");
 }
 
@@ -574,22 +610,31 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
     std::vector<double> fsave((size_t)(3 * n));
     s = komd_get_sites(c, nullptr, nullptr, fsave.data());
     if (s) return s;
-    unsigned long long cnt0[2];
-    HIPCALL(c, hipMemcpy(cnt0, d.counters, sizeof(cnt0), hipMemcpyDeviceToHost), "komd_measure_observables");
+    int64_t cnt0[2];
+    s = read_counters(c, cnt0, false);
+    if (s) return s;
     HIPCALL(c, launch_com_wrap_key(c, false), "komd_measure_observables");
     HIPCALL(c, launch_bin(c), "komd_measure_observables");
     double* out_dev = d.scal + 8;
     double vals[8];
     const int atm = atm_code(kind3b);
-    HIPCALL(c, launch_cell(c, true, atm, true, false, nullptr, nullptr, nullptr), "komd_measure_observables");
-    const int pb = cell_blocks(c);
-    HIPCALL(c, launch_reduce_parts(c, d.red, pb, 3, out_dev), "komd_measure_observables");
+    const int nl = atm ? atm : (kind3b ? 1 : 0);
+    if (nl) HIPCALL(c, ensure_nl(c, nl == 2), "komd_measure_observables");
+    HIPCALL(c, launch_cell(c, true, nl, true, false), "komd_measure_observables");
+    HIPCALL(c, launch_reduce_parts(c, d.red, cell_blocks(c), 2, out_dev), "komd_measure_observables");
     s = sync_check(c, "komd_measure_observables");
     if (s) return s;
-    HIPCALL(c, hipMemcpy(vals, out_dev, 3 * sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+    HIPCALL(c, hipMemcpy(vals, out_dev, 2 * sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
     out->u_lj = vals[0];
     out->virial = vals[1];
-    out->u_3b = vals[2];
+    if (atm) {
+        HIPCALL(c, launch_atm(c, atm, true, nullptr, nullptr, nullptr), "komd_measure_observables");
+        HIPCALL(c, launch_reduce_parts(c, d.red, atm_blocks(c), 1, out_dev), "komd_measure_observables");
+        s = sync_check(c, "komd_measure_observables");
+        if (s) return s;
+        HIPCALL(c, hipMemcpy(vals, out_dev, sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+        out->u_3b = vals[0];
+    }
     if (c->nbonds + c->nangles > 0) {
         HIPCALL(c, hipMemset(d.red, 0, 512 * sizeof(double)), "komd_measure_observables");
         HIPCALL(c, launch_bonded(c, true), "komd_measure_observables");
@@ -610,10 +655,11 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
         HIPCALL(c, hipMemcpy(vals, out_dev, sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
         out->u_3b = vals[0];
     }
-    unsigned long long cnt1[2];
-    HIPCALL(c, hipMemcpy(cnt1, d.counters, sizeof(cnt1), hipMemcpyDeviceToHost), "komd_measure_observables");
-    out->pairs = (int64_t)(cnt1[0] - cnt0[0]);
-    out->triplets = (int64_t)(cnt1[1] - cnt0[1]) / 3;  // every triplet is visited by its 3 members
+    int64_t cnt1[2];
+    s = read_counters(c, cnt1, false);
+    if (s) return s;
+    out->pairs = cnt1[0] - cnt0[0];
+    out->triplets = (cnt1[1] - cnt0[1]) / 3;  // every triplet is visited by its 3 members
     // kinetic energy and temperature
     HIPCALL(c, launch_temperature(c), "komd_measure_observables");
     s = sync_check(c, "komd_measure_observables");
@@ -651,14 +697,13 @@ komd_status komd_timing_get(komd_ctx* c, const char* name, double* total_ms, int
 
 komd_status komd_counters(komd_ctx* c, int64_t* pairs, int64_t* triplets, int32_t reset) {
     if (!c) return KOMD_ERR_ARG;
-    unsigned long long v[2] = {0, 0};
-    if (c->d.counters) {
-        HIPCALL(c, hipStreamSynchronize(c->stream), "komd_counters");
-        HIPCALL(c, hipMemcpy(v, c->d.counters, sizeof(v), hipMemcpyDeviceToHost), "komd_counters");
-        if (reset) HIPCALL(c, hipMemset(c->d.counters, 0, sizeof(v)), "komd_counters");
+    int64_t v[2] = {0, 0};
+    if (c->have_sites) {
+        komd_status s = read_counters(c, v, reset != 0);
+        if (s) return s;
     }
-    if (pairs) *pairs = (int64_t)v[0];
-    if (triplets) *triplets = (int64_t)v[1] / 3;  // every triplet is visited by its 3 members
+    if (pairs) *pairs = v[0];
+    if (triplets) *triplets = v[1] / 3;  // every triplet is visited by its 3 members
     return KOMD_OK;
 }
 

@@ -10,6 +10,9 @@
 //   spos[N] float4 cell-sorted FP32 position relative to the cell centre, .w = site index bits
 //   sprm[N] float2 cell-sorted (sigma, sqrt(eps)) when the sites are not uniform
 //   sid[N]  u32    cell-sorted id class when molecules have several sites
+//   nl[N][96] float4  full neighbour list of every site, cell-sorted order:
+//                  (r_j - r_i) in FP32, .w = j | image code << 26; nlc[N] its length,
+//                  nld[N][96] u8 neighbour cell direction (reference-mode ATM only)
 //   esite[N] f64   per-site three-body energy share (training targets)
 #pragma once
 #include <hip/hip_runtime.h>
@@ -56,13 +59,19 @@ struct DevState {
     float4* spos = nullptr;
     float2* sprm = nullptr;
     uint32_t* sid = nullptr;
+    // full neighbour lists (three-body and surrogate passes)
+    float4* nl = nullptr;
+    uint32_t* nlc = nullptr;
+    uint8_t* nld = nullptr;
     // bonded
     uint32_t *bonds = nullptr, *angles = nullptr;
     // reductions and flags
     double* red = nullptr;        // partials (see k_container.hip)
     double* scal = nullptr;       // device scalars: [0] mv2, [1] count, [2] beta, ...
     uint32_t* flags = nullptr;
-    unsigned long long* counters = nullptr;  // [0] pairs, [1] triplets
+    unsigned long long* counters = nullptr;  // unused legacy slot
+    unsigned long long* cnt_pair = nullptr;  // [cell blocks] in-range pairs per workgroup slot
+    unsigned long long* cnt_trip = nullptr;  // [ATM blocks] triplet visits per workgroup slot
     double* esite = nullptr;      // per-site energy scratch [N]
 };
 
@@ -133,12 +142,14 @@ hipError_t launch_temperature(komd_ctx* c);     // kinetic partials -> scal[0..1
 hipError_t launch_rescale(komd_ctx* c, double t0);    // beta from scal, v *= beta
 hipError_t launch_beta_only(komd_ctx* c);             // beta = sqrt(run_t0/T) into scal[2]
 hipError_t launch_reduce_parts(komd_ctx* c, const double* part, int nparts, int nslots, double* out_dev);
-// k_cell.hip: LJ, clamp and ATM on cell tiles.  atm: 0 off, 1 physical, 2 reference.
-// limit: clamp the two-body force before the three-body term is added.
-// lam/dlam: hybrid weights (nullptr otherwise); esite: per-site ATM energy out (optional).
-hipError_t launch_cell(komd_ctx* c, bool lj, int atm, bool energy, bool limit, const float* lam, const float* dlam,
-                       double* esite);
+// k_cell.hip.  launch_cell: LJ (+ clamp) on cell tiles; nl: 0 no neighbour list, 1 write
+// the full lists, 2 lists + cell directions.  launch_atm: per-site ATM from the lists;
+// atm: 1 physical, 2 reference; lam/dlam: hybrid weights (nullptr otherwise); esite:
+// per-site energy share out (optional).
+hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit);
+hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite);
 int cell_blocks(komd_ctx* c);
+int atm_blocks(komd_ctx* c);
 // k_bonded.hip
 hipError_t launch_bonded(komd_ctx* c, bool energy);
 hipError_t launch_limit(komd_ctx* c);
