@@ -1,24 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the KoMD force hot path on one MI355X.
 
-Workload (the exact three-body argon configuration): 32,000 argon atoms, FCC
-20^3 x 4 at reduced density 0.8, T* = 0.8, LJ 12-6 (rc 2.5) + Axilrod-Teller-
-Muto (nu 0.073) over every triplet with all three distances below 2.5, velocity
-Verlet dt = 0.005, NVE.  A step is one pass of the hot path: kick-drift and
-wrap, counting-sort binning, fused LJ + ATM cell-tile forces, kick.
+Default workload (the exact three-body argon configuration): 32,000 argon
+atoms, FCC 20^3 x 4 at reduced density 0.8, T* = 0.8, LJ 12-6 (rc 2.5) +
+Axilrod-Teller-Muto (nu 0.073) over every triplet with all three distances
+below 2.5, velocity Verlet dt = 0.005, NVE.  A step is one pass of the hot
+path: kick-drift and wrap, counting-sort binning, cell-tile LJ with the full
+neighbour lists, per-site ATM, kick.
 
 Prints one JSON line:
   value        atom-timesteps/s of K graph-replayed steps (inputs resident in HBM)
-  roofline     the fused LJ+ATM kernel: algorithmic FP32 FLOPs (26 per unique
-               in-range pair + 147 per unique in-range triplet, counted by the
-               kernel and pinned to the oracle's counts in the tests) divided by
-               its average launch time measured with HIP events on the library's
-               stream over a second pass of the same K steps
+  roofline     the dominant kernel (per-site ATM): algorithmic FP32 FLOPs
+               (147 per unique in-range triplet, counted by the kernel and pinned
+               to the oracle's count in the tests) over its average launch time,
+               measured with HIP events on the library's stream over a second
+               pass of the same K steps
   cpu_baseline the FP64 OpenMP restatement of the reference algorithm (C08 cell
                pairs, ATM_NOLIST triplet walk, per-interaction atomics), timed on
                this host on a bounded sample of the same workload
-  force_error  max / RMS |F_gpu - F_fp64| / RMS(F_fp64) on the start snapshot,
-               total force and three-body force separately
+  force_error  max / RMS |F_gpu - F_fp64| / RMS(F_fp64) on an equilibrated
+               snapshot, total force and three-body force separately
+  energy_drift NVE drift of the total energy per atom over --drift-steps steps
 """
 from __future__ import annotations
 
@@ -38,7 +40,7 @@ NU = 0.073
 FLOP_PAIR = 26      # per unique in-range LJ pair
 FLOP_TRIPLET = 147  # per unique in-range ATM triplet
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "atm_32000_traffic.json")
 
 
 def parse():
@@ -46,9 +48,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--atoms", type=int, default=32000, choices=[32000, 256000],
+                    help="32000: FCC 20^3 x 4 (default); 256000: FCC 40^3 x 4")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 force-error check")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--equil-steps", type=int, default=400, help="steps before the force-error snapshot")
+    ap.add_argument("--drift-steps", type=int, default=10000, help="NVE steps for the energy drift (0: skip)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -58,67 +66,119 @@ def rel_err(a, b):
     return {"max": float(d.max() / scale), "rms": float(np.sqrt(np.mean(d * d)) / scale)}
 
 
+def copy_calibration(mib: int):
+    """A plain device copy of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE read on
+    it give the factor that turns those counters into HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the force path is single-GPU here (spatial decomposition "
               "over several GPUs is described in DESIGN.md but not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        return
 
     import komd_amd as K
     from komd_amd import systems as S
 
-    sysm = S.config_atm_32000()
+    sysm = S.config_atm_32000() if args.atoms == 32000 else S.config_atm_256000()
     n = sysm.n
     dev = K.device_name()
     ctx = K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU)
+    ctx.compute_forces(K.ATM_EXACT)
 
-    # ---- force error against FP64 on the start snapshot (oracle = checker only)
+    # ---- force error against FP64 on an equilibrated snapshot (oracle = checker only)
     force_error = None
     if not args.no_check:
         import oracle
-        r, b = oracle.update_container(sysm)
-        f_lj = oracle.lj_c08(sysm, r, b, nthreads=os.cpu_count() or 1)
-        f_3b, _, _ = oracle.atm_complete(sysm, r, b, NU, coverage=0, sign=-1.0, nthreads=os.cpu_count() or 1)
+        ctx.run(args.equil_steps, kind3b=K.ATM_EXACT)
+        r_snap, v_snap, _ = ctx.get_state()
+        snap = sysm.copy()
+        snap.r, snap.v = r_snap, v_snap
+        threads = os.cpu_count() or 1
+        r, b = oracle.update_container(snap)
+        f_lj = oracle.lj_c08(snap, r, b, nthreads=threads)
+        f_3b, _, _ = oracle.atm_complete(snap, r, b, NU, coverage=0, sign=-1.0, nthreads=threads)
         ctx.compute_forces(K.ATM_EXACT)
         f_tot = ctx.forces()
         ctx.update_container()
         ctx.forces_3b(K.ATM_EXACT)
         f_atm = ctx.forces()
         force_error = {"total": rel_err(f_tot, f_lj + f_3b), "three_body": rel_err(f_atm, f_3b),
-                       "reference": "FP64 oracle, same snapshot"}
+                       "snapshot": f"after {args.equil_steps} NVE steps from the lattice",
+                       "reference": "FP64 oracle (C restatement) on the same positions"}
+        ctx.compute_forces(K.ATM_EXACT)
 
     # ---- throughput: K graph-replayed steps, inputs resident in HBM
-    ctx.compute_forces(K.ATM_EXACT)
     ctx.run(args.warmup, kind3b=K.ATM_EXACT)
     ctx.counters(reset=True)
     t0 = time.perf_counter()
     ctx.run(args.steps, kind3b=K.ATM_EXACT)   # synchronises at the end
     t1 = time.perf_counter()
-    pairs, trips = ctx.counters(reset=True)
+    ctx.counters(reset=True)
     dt = t1 - t0
     value = n * args.steps / dt
 
     # ---- per-kernel device time with HIP events over a second pass of K steps
     ctx.timing(True)
     ctx.run(args.steps, kind3b=K.ATM_EXACT)
-    kms, kl = ctx.timer("atm")
-    pms, pl = ctx.timer("cell_lj_nl")
-    p2, t2 = ctx.counters(reset=True)
+    pairs, trips = ctx.counters(reset=True)
     kernels = {}
     for name in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj_nl", "atm", "kick"):
         ms, cnt = ctx.timer(name)
-        kernels[name] = round(ms / max(cnt, 1) * 1e3, 2)
+        kernels[name] = {"us_per_launch": round(ms / max(cnt, 1) * 1e3, 2), "launches": cnt}
+    atm_ms, atm_n = ctx.timer("atm")
+    cell_ms, cell_n = ctx.timer("cell_lj_nl")
     ctx.timing(False)
-    flops = FLOP_PAIR * p2 + FLOP_TRIPLET * t2
-    achieved = flops / (kms * 1e-3) / 1e12
-    roofline = {"kernel": "k_cell<LJ+ATM> (fused cell-tile LJ 12-6 + ATM)", "bound": "valu",
-                "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "avg_launch_us": round(kms / max(kl, 1) * 1e3, 2),
-                "algorithmic_flop_per_launch": int(round(flops / max(kl, 1))),
-                "unique_pairs_per_step": int(p2 // max(kl, 1)), "unique_triplets_per_step": int(t2 // max(kl, 1)),
-                "timing": "HIP events on the library stream, second pass of the same steps"}
+    flop_atm = FLOP_TRIPLET * trips
+    achieved = flop_atm / (atm_ms * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.atoms == 32000:
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    roofline = {
+        "kernel": "k_atm (per-site Axilrod-Teller-Muto, FP32 VALU)", "bound": "valu",
+        "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+        "avg_launch_us": round(atm_ms / max(atm_n, 1) * 1e3, 2),
+        "algorithmic_flop_per_launch": int(round(flop_atm / max(atm_n, 1))),
+        "unique_triplets_per_step": int(trips // max(atm_n, 1)),
+        "peak_note": "MI355X FP32 vector peak 157.3 TFLOP/s counts packed FMA; the kernels are built "
+                     "without packed FP32, whose own ceiling is 78.6 TFLOP/s",
+        "timing": "HIP events on the library stream, second pass of the same steps",
+        "second_kernel": {"kernel": "k_cell (LJ 12-6 + neighbour lists)",
+                          "achieved_tflops": round(FLOP_PAIR * pairs / (cell_ms * 1e-3) / 1e12, 3),
+                          "avg_launch_us": round(cell_ms / max(cell_n, 1) * 1e3, 2),
+                          "unique_pairs_per_step": int(pairs // max(cell_n, 1))},
+    }
+
+    # ---- NVE energy drift over drift_steps (E per atom, linear fit against time)
+    drift = None
+    if args.drift_steps > 0:
+        chunk = max(1, args.drift_steps // 10)
+        ts, es = [], []
+        for k in range(11):
+            o = ctx.observables(K.ATM_EXACT)
+            ts.append(k * chunk * 0.005)
+            es.append((o["kinetic"] + o["u_lj"] + o["u_3b"]) / n)
+            if k < 10:
+                ctx.run(chunk, kind3b=K.ATM_EXACT)
+        slope = float(np.polyfit(ts, es, 1)[0])
+        drift = {"steps": 10 * chunk, "slope_e_per_atom_per_tau": slope, "e0_per_atom": es[0],
+                 "e_end_per_atom": es[-1], "max_dev_per_atom": float(np.max(np.abs(np.array(es) - es[0]))),
+                 "note": "LJ and ATM truncated at rc without shift, as the reference's energy "
+                         "(LJ12_6_Sensor.cpp:76): pairs and triplets crossing rc contribute energy steps"}
 
     # ---- CPU baseline: FP64 OpenMP restatement of the reference loop, bounded sample
     cpu = None
@@ -131,23 +191,25 @@ def main():
         c1 = time.perf_counter()
         cpu = {"value": round(n * args.cpu_steps / (c1 - c0), 1), "unit": "atom-timesteps/s", "cores": threads,
                "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the 32,000-atom LJ+ATM workload after 1 warm step: C08 cell "
+               "sample": f"{args.cpu_steps} steps of the {n}-atom LJ+ATM workload after 1 warm step: C08 cell "
                          f"pairs (LinkedCells.cpp:164-224) + ATM_NOLIST triplet walk (ATM_NOLIST.cpp:20-159) with "
-                         f"FP64 atomics, {threads} OpenMP threads, {c1 - c0:.2f} s"}
+                         f"FP64 atomics, {threads} OpenMP threads, {c1 - c0:.2f} s wall"}
 
+    cells = "20^3" if n == 32000 else "40^3"
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "atom-timesteps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (pair/triplet arithmetic), f64 (positions, force accumulation, reductions)",
         "data": "synthetic: seeded FCC argon lattice, Maxwell-Boltzmann velocities",
-        "config": {"workload": "exact 3-body argon, 32,000 atoms (FCC 20^3 x 4), rho 0.8, T* 0.8, LJ 12-6 rc 2.5 "
-                               "+ ATM nu 0.073 over every triplet within rc, dt 0.005, NVE, 1 GPU",
+        "config": {"workload": f"exact 3-body argon, {n:,} atoms (FCC {cells} x 4), rho 0.8, T* 0.8, LJ 12-6 "
+                               f"rc 2.5 + ATM nu 0.073 over every triplet within rc, dt 0.005, NVE, 1 GPU",
                    "n_atoms": n, "device": dev, "path": "exact ATM (physical sign and complete triplet set)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "force_error": force_error,
-        "kernels_us_per_launch": kernels,
+        "energy_drift": drift,
+        "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
 

@@ -210,7 +210,8 @@ komd_status komd_device_name(char* buf, int32_t len) {
     int dev = 0;
     hipDeviceProp_t p;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return KOMD_ERR_HIP;
-    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    snprintf(buf, (size_t)len, "%s%s%s, %d CUs, %.0f GB", p.name, p.name[0] ? " " : "", p.gcnArchName,
+             p.multiProcessorCount, (double)p.totalGlobalMem / 1e9);
     return KOMD_OK;
 }
 
