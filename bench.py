@@ -21,6 +21,11 @@ Prints one JSON line:
   force_error  max / RMS |F_gpu - F_fp64| / RMS(F_fp64) on an equilibrated
                snapshot, total force and three-body force separately
   energy_drift NVE drift of the total energy per atom over --drift-steps steps
+  learned      the learned three-body path (descriptor-MLP surrogate, weights in
+               komd_amd/weights/) at 32,000 and 256,000 atoms: throughput beside the
+               exact path on the same state, its three-body force error against the
+               exact ATM (and that of the mean-field ATM2B scaling and of LJ alone), the
+               surrogate kernels' times and its NVE energy drift
 """
 from __future__ import annotations
 
@@ -41,6 +46,8 @@ FLOP_PAIR = 26      # per unique in-range LJ pair
 FLOP_TRIPLET = 147  # per unique in-range ATM triplet
 FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "atm_32000_traffic.json")
+WEIGHTS = os.path.join(ROOT, "komd_amd", "weights", "atm_surrogate.npy")
+RHO = 0.8
 
 
 def parse():
@@ -55,6 +62,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--equil-steps", type=int, default=400, help="steps before the force-error snapshot")
     ap.add_argument("--drift-steps", type=int, default=10000, help="NVE steps for the energy drift (0: skip)")
+    ap.add_argument("--learned-atoms", default="32000,256000",
+                    help="comma list of system sizes for the learned leg ('' to skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -64,6 +73,78 @@ def rel_err(a, b):
     scale = float(np.sqrt(np.mean(b * b)))
     d = np.linalg.norm(a - b, axis=1)
     return {"max": float(d.max() / scale), "rms": float(np.sqrt(np.mean(d * d)) / scale)}
+
+
+def drift_of(ctx, K, n, kind, steps):
+    """Slope of E_total / N against time over `steps` NVE steps (11 samples)."""
+    chunk = max(1, steps // 10)
+    ts, es = [], []
+    for k in range(11):
+        o = ctx.observables(kind)
+        ts.append(k * chunk * 0.005)
+        es.append((o["kinetic"] + o["u_lj"] + o["u_3b"]) / n)
+        if k < 10:
+            ctx.run(chunk, kind3b=kind)
+    return {"steps": 10 * chunk, "slope_e_per_atom_per_tau": float(np.polyfit(ts, es, 1)[0]),
+            "max_dev_per_atom": float(np.max(np.abs(np.array(es) - es[0])))}
+
+
+def learned_leg(n_atoms, args):
+    """Exact ATM and the surrogate on the same equilibrated state of an argon system."""
+    import komd_amd as K
+    from komd_amd import systems as S
+    sysm = S.config_atm_32000() if n_atoms == 32000 else S.config_atm_256000()
+    n = sysm.n
+    out = {"n_atoms": n}
+    with K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU) as ctx:
+        ctx.surrogate_set_params(np.load(WEIGHTS))
+        ctx.compute_forces(K.ATM_EXACT)
+        ctx.run(args.equil_steps, kind3b=K.ATM_EXACT)
+        r0, v0, _ = ctx.get_state()
+        # three-body force error of the surrogate, of ATM2B and of LJ alone against the exact ATM
+        rel, mx = ctx.surrogate_judge()
+        ctx.update_container()
+        ctx.forces_3b(K.ATM_EXACT)
+        f_atm = ctx.forces()
+        ctx.update_container()
+        ctx.forces_pair()
+        f_lj = ctx.forces()
+        atm2b = (1.0 - 2.0 * NU * RHO / 3.0) - 1.0   # ATM2B.cpp:9-20: LJ scaled by 1 - 2 nu rho / (3 eps sig^6)
+        scale = float(np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1))))
+        err_2b = float(np.sqrt(np.mean(np.sum((atm2b * f_lj - f_atm) ** 2, axis=1)))) / scale
+        out["three_body_force_error"] = {
+            "surrogate": {"rms": rel, "max": mx}, "atm2b_mean_field": {"rms": err_2b}, "lj_only": {"rms": 1.0},
+            "reference": "exact ATM (FP32 arithmetic, FP64 accumulation; 1.6e-6 RMS from the FP64 oracle) "
+                         f"after {args.equil_steps} NVE steps from the lattice"}
+        legs = {}
+        for kind, name in ((K.ATM_EXACT, "exact"), (K.SURROGATE, "surrogate")):
+            ctx.set_state(r0, v0)
+            ctx.compute_forces(kind)
+            ctx.run(args.warmup, kind3b=kind)
+            t0 = time.perf_counter()
+            ctx.run(args.steps, kind3b=kind)
+            dt = time.perf_counter() - t0
+            ctx.timing(True)
+            ctx.run(min(args.steps, 200), kind3b=kind)
+            kern = {}
+            for k in ("cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
+                ms, cnt = ctx.timer(k)
+                if cnt:
+                    kern[k] = round(ms / cnt * 1e3, 2)
+            ctx.timing(False)
+            legs[name] = {"value": round(n * args.steps / dt, 1), "ms_per_step": round(dt / args.steps * 1e3, 4),
+                          "kernels_us": kern}
+        out["exact"], out["surrogate"] = legs["exact"], legs["surrogate"]
+        out["speedup_surrogate_vs_exact"] = round(legs["surrogate"]["value"] / legs["exact"]["value"], 3)
+        out["three_body_part_us"] = {
+            "exact": legs["exact"]["kernels_us"].get("atm"),
+            "surrogate": round(sum(legs["surrogate"]["kernels_us"].get(k, 0.0)
+                                   for k in ("sur_desc", "sur_mlp", "sur_force", "sur_apply")), 2)}
+        if args.drift_steps > 0 and n_atoms == 32000:
+            ctx.set_state(r0, v0)
+            ctx.compute_forces(K.SURROGATE)
+            out["energy_drift_surrogate"] = drift_of(ctx, K, n, K.SURROGATE, args.drift_steps)
+    return out
 
 
 def copy_calibration(mib: int):
@@ -166,19 +247,15 @@ def main():
     # ---- NVE energy drift over drift_steps (E per atom, linear fit against time)
     drift = None
     if args.drift_steps > 0:
-        chunk = max(1, args.drift_steps // 10)
-        ts, es = [], []
-        for k in range(11):
-            o = ctx.observables(K.ATM_EXACT)
-            ts.append(k * chunk * 0.005)
-            es.append((o["kinetic"] + o["u_lj"] + o["u_3b"]) / n)
-            if k < 10:
-                ctx.run(chunk, kind3b=K.ATM_EXACT)
-        slope = float(np.polyfit(ts, es, 1)[0])
-        drift = {"steps": 10 * chunk, "slope_e_per_atom_per_tau": slope, "e0_per_atom": es[0],
-                 "e_end_per_atom": es[-1], "max_dev_per_atom": float(np.max(np.abs(np.array(es) - es[0]))),
-                 "note": "LJ and ATM truncated at rc without shift, as the reference's energy "
-                         "(LJ12_6_Sensor.cpp:76): pairs and triplets crossing rc contribute energy steps"}
+        drift = drift_of(ctx, K, n, K.ATM_EXACT, args.drift_steps)
+        drift["note"] = ("LJ and ATM truncated at rc without shift, as the reference's energy "
+                         "(LJ12_6_Sensor.cpp:76): pairs and triplets crossing rc contribute energy steps")
+
+    # ---- learned three-body path beside the exact one
+    learned = {}
+    if args.learned_atoms and os.path.exists(WEIGHTS):
+        for na in [int(x) for x in args.learned_atoms.split(",") if x]:
+            learned[str(na)] = learned_leg(na, args)
 
     # ---- CPU baseline: FP64 OpenMP restatement of the reference loop, bounded sample
     cpu = None
@@ -210,6 +287,7 @@ def main():
         "force_error": force_error,
         "energy_drift": drift,
         "kernels": kernels,
+        "learned": learned or None,
     }
     print(json.dumps(out), flush=True)
 

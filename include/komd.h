@@ -177,35 +177,68 @@ komd_status komd_timing_get(komd_ctx* ctx, const char* name, double* total_ms, i
 komd_status komd_counters(komd_ctx* ctx, int64_t* pairs, int64_t* triplets, int32_t reset);
 
 /* ---- learned three-body surrogate ------------------------------------- */
-/* Descriptor-MLP: 32 radial + 16 angular descriptors within cutoff -> 64 -> 64 -> 1,
- * SiLU, BF16 weights with FP32 accumulation, per-site energy summed; forces = -dE/dr
- * by a hand-written backward.  No reference counterpart. */
+/* Descriptor-MLP surrogate of the ATM energy (no reference counterpart; the nearest
+ * reference method is the mean-field ATM2B scaling, potentials/ATM2B.cpp:9-20).
+ * Per site i, within the cutoff:
+ *   32 radial descriptors  sum_j exp(-((r_ij - mu_k)/w)^2) fc(r_ij),  mu_k = 0.85 + k*1.6/31, w = 1.6/31
+ *   16 angular invariants  <T_a^L, T_b^L>, T_a^L = sum_j psi_a(r_ij) u_ij^(x L), L = 1, 2, 3,
+ *                          psi_a(r) = exp(-((r - nu_a)/0.35)^2) fc(r) / r^3, nu = 1.0, 1.35, 1.75, 2.15
+ *   fc(r) = (cos(pi r / rc) + 1) / 2
+ * x = (D - shift) * scale -> 64 -> 64 -> 1 with SiLU; BF16 MFMA operands, FP32 accumulation.
+ * The three-body energy is sum_i E_i; forces are -dE/dr by a hand-written backward. */
 #define KOMD_SUR_NDESC 48
 #define KOMD_SUR_H1 64
 #define KOMD_SUR_H2 64
-/* Number of float parameters: W1[64x48], b1[64], W2[64x64], b2[64], w3[64], b3[1],
- * then the 48 descriptor shifts and 48 descriptor scales (input standardisation). */
+/* Number of float parameters: W1[64][48], b1[64], W2[64][64], b2[64], w3[64], b3[1],
+ * then the 48 descriptor shifts and the 48 descriptor scales. */
 #define KOMD_SUR_NPARAM (64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48)
 komd_status komd_surrogate_set_params(komd_ctx* ctx, const float* params, int64_t n);
 komd_status komd_surrogate_get_params(komd_ctx* ctx, float* params, int64_t n);
-/* Descriptors of every site of the current configuration ([n][48], FP32). */
+/* Raw (unstandardised) descriptors of every site of the current configuration,
+ * [n][48] FP32 in site order.  Rebuilds the container and neighbour lists. */
 komd_status komd_surrogate_descriptors(komd_ctx* ctx, float* desc_out);
-/* Per-site surrogate energies ([n], FP32) of the current configuration. */
+/* Per-site surrogate energies E_i ([n], FP32, site order) of the current configuration. */
 komd_status komd_surrogate_energies(komd_ctx* ctx, float* e_out);
-/* Exact per-site ATM energy shares V/3 ([n], FP64) of the current configuration. */
+/* Exact per-site ATM energy shares V/3 ([n], FP64, site order) of the current configuration. */
 komd_status komd_atm_site_energies(komd_ctx* ctx, double* e_out);
-/* Training set on the device: append the current configuration's descriptors and
- * exact per-site ATM energies; clear it. */
+
+/* Training set on the device: komd_train_add_snapshot stores the current positions with
+ * their exact (physical) ATM forces and per-site energy shares; positions, velocities and
+ * forces of the context are left as they were. */
 komd_status komd_train_add_snapshot(komd_ctx* ctx);
 komd_status komd_train_clear(komd_ctx* ctx);
-komd_status komd_train_size(komd_ctx* ctx, int64_t* n_samples);
-/* Sets descriptor standardisation from the stored samples (mean, 1/std). */
+komd_status komd_train_size(komd_ctx* ctx, int64_t* n_snapshots);
+/* Target statistics of the stored set: [0] mean E_i, [1] std E_i, [2] RMS force component. */
+komd_status komd_train_stats(komd_ctx* ctx, double out[3]);
+/* Sets shift = mean and scale = 1/std of every descriptor over the stored snapshots
+ * (allocates zero parameters first when none are set). */
 komd_status komd_train_standardize(komd_ctx* ctx);
-/* One epoch of minibatch Adam on per-site energies with the project's own BF16
- * MFMA forward/backward kernels.  Returns the epoch's mean squared error. */
-komd_status komd_train_epoch(komd_ctx* ctx, int32_t batch, float lr, uint64_t seed, double* mse);
-/* Hybrid region: exact ATM for |x - center| < half_width along axis, surrogate
- * beyond half_width + blend, cos^2 weight in between (AdResS.cpp:60-74 shape). */
+/* Loss L = wE/N sum_i (E_i - E*_i)^2 + wF/(3N) sum_i |F_i - F*_i|^2 of one stored snapshot
+ * and its gradient with respect to the parameters (shift/scale entries 0). */
+komd_status komd_train_gradient(komd_ctx* ctx, int64_t snapshot, float w_energy, float w_force, float* grad,
+                                double* loss);
+/* One epoch: every stored snapshot once, in an order drawn from seed, one Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8) per snapshot with learning rate lr.  loss: mean
+ * of the per-snapshot losses before their steps.  Positions and forces of the context
+ * are left as they were. */
+komd_status komd_train_epoch(komd_ctx* ctx, float lr, float w_energy, float w_force, uint64_t seed,
+                             double* loss);
+/* Exact ATM as judge: three-body force error of the surrogate at the current
+ * configuration, out[0] = RMS |F_sur - F_atm| / RMS |F_atm|, out[1] = max |F_sur - F_atm| /
+ * RMS |F_atm|.  Positions and forces are left as they were. */
+komd_status komd_surrogate_judge(komd_ctx* ctx, double out[2]);
+/* komd_run with KOMD_3B_SURROGATE or KOMD_3B_HYBRID judges every `every` steps (0: never);
+ * when the relative RMS error exceeds max_rel_rms the run continues with the exact ATM. */
+komd_status komd_set_judge(komd_ctx* ctx, int64_t every, double max_rel_rms);
+/* Judge history since the last komd_set_judge: checks, last and worst relative RMS error,
+ * and whether the run fell back to the exact ATM. */
+komd_status komd_judge_report(komd_ctx* ctx, int64_t* checks, double* last_rel_rms, double* worst_rel_rms,
+                              int32_t* fell_back);
+/* Hybrid region for KOMD_3B_HYBRID: weight l = 1 (exact ATM) for |x - center| < half_width
+ * along axis (minimum image), cos^2 descent to 0 over `blend`, surrogate weight 1 - l.
+ * A triplet's ATM energy is weighted by the mean l of its members, a site's surrogate
+ * energy by 1 - l_i (the AdResS-style weighting of plugins/ADR/AdResS.cpp:60-74, applied
+ * to energies so that the blended force is conservative). */
 komd_status komd_set_hybrid(komd_ctx* ctx, int32_t axis, double center, double half_width, double blend);
 
 #ifdef __cplusplus

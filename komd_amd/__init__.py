@@ -91,7 +91,12 @@ _SIGNATURES = {
     "komd_train_clear": (C.c_int, [_P]),
     "komd_train_size": (C.c_int, [_P, _P]),
     "komd_train_standardize": (C.c_int, [_P]),
-    "komd_train_epoch": (C.c_int, [_P, C.c_int32, C.c_float, C.c_uint64, _P]),
+    "komd_train_stats": (C.c_int, [_P, _P]),
+    "komd_train_gradient": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, _P, _P]),
+    "komd_train_epoch": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_uint64, _P]),
+    "komd_surrogate_judge": (C.c_int, [_P, _P]),
+    "komd_set_judge": (C.c_int, [_P, C.c_int64, C.c_double]),
+    "komd_judge_report": (C.c_int, [_P, _P, _P, _P, _P]),
     "komd_set_hybrid": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_double]),
 }
 
@@ -337,10 +342,36 @@ class Context:
     def train_standardize(self):
         self._check(self.lib.komd_train_standardize(self._ctx))
 
-    def train_epoch(self, batch: int, lr: float, seed: int) -> float:
-        mse = C.c_double()
-        self._check(self.lib.komd_train_epoch(self._ctx, batch, lr, seed, C.byref(mse)))
-        return mse.value
+    def train_stats(self):
+        """(mean E_i, std E_i, RMS force component) of the stored targets."""
+        out = np.zeros(3)
+        self._check(self.lib.komd_train_stats(self._ctx, _p(out)))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def train_gradient(self, snapshot: int, w_energy: float, w_force: float):
+        g = np.zeros(SUR_NPARAM, np.float32)
+        loss = C.c_double(0)
+        self._check(self.lib.komd_train_gradient(self._ctx, snapshot, w_energy, w_force, _p(g), C.byref(loss)))
+        return loss.value, g
+
+    def train_epoch(self, lr: float, w_energy: float, w_force: float, seed: int) -> float:
+        loss = C.c_double(0)
+        self._check(self.lib.komd_train_epoch(self._ctx, lr, w_energy, w_force, seed, C.byref(loss)))
+        return loss.value
+
+    def surrogate_judge(self):
+        out = np.zeros(2)
+        self._check(self.lib.komd_surrogate_judge(self._ctx, _p(out)))
+        return float(out[0]), float(out[1])
+
+    def set_judge(self, every: int, max_rel_rms: float):
+        self._check(self.lib.komd_set_judge(self._ctx, every, max_rel_rms))
+
+    def judge_report(self):
+        n, last, worst, fell = C.c_int64(0), C.c_double(0), C.c_double(0), C.c_int32(0)
+        self._check(self.lib.komd_judge_report(self._ctx, C.byref(n), C.byref(last), C.byref(worst), C.byref(fell)))
+        return {"checks": n.value, "last_rel_rms": last.value, "worst_rel_rms": worst.value,
+                "fell_back": bool(fell.value)}
 
     def set_hybrid(self, axis: int, center: float, half_width: float, blend: float):
         self._check(self.lib.komd_set_hybrid(self._ctx, axis, center, half_width, blend))

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_repack(
     uint32_t* __restrict__ sorted, const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ rz, float4* __restrict__ spos, const double* __restrict__ eps,
     const double* __restrict__ sig, float2* __restrict__ sprm, const uint32_t* __restrict__ idcls,
-    uint32_t* __restrict__ sid, Grid g) {
+    uint32_t* __restrict__ sid, uint32_t* __restrict__ inv, Grid g) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = key[i];
@@ -159,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_repack(
     for (uint32_t t = s; t < e; t++) rank += (tmp[t] < (uint32_t)i) ? 1u : 0u;
     const uint32_t p = s + rank;
     sorted[p] = (uint32_t)i;
+    inv[i] = p;
     const int ix = (int)(k % (uint32_t)g.nx);
     const int iy = (int)((k / (uint32_t)g.nx) % (uint32_t)g.ny);
     const int iz = (int)(k / (uint32_t)(g.nx * g.ny));
@@ -327,7 +328,7 @@ hipError_t launch_bin(komd_ctx* c) {
         ScopedTimer tm(c, "bin_sort");
         k_sort_repack<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
             n, d.key, d.start, d.tmp, d.sorted, d.rx, d.ry, d.rz, d.spos, d.eps, d.sig, c->uniform ? nullptr : d.sprm,
-            d.idcls, c->multisite ? d.sid : nullptr, c->grid);
+            d.idcls, c->multisite ? d.sid : nullptr, d.inv, c->grid);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     return hipSuccess;

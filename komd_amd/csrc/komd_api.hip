@@ -11,9 +11,6 @@
 
 namespace komd {
 hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev);
-hipError_t surrogate_energies(komd_ctx* c, float* e_dev);
-hipError_t surrogate_descriptors(komd_ctx* c, float* desc_dev);
-hipError_t hybrid_forces(komd_ctx* c, bool energy);
 
 void timer_begin(komd_ctx* c, const char* name, hipEvent_t* a, hipEvent_t* b) {
     (void)name;
@@ -67,7 +64,7 @@ hipError_t dalloc(T** p, size_t count) {
 void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
-                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.red, d.cnt_pair, d.cnt_trip,
+                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.red, d.cnt_pair, d.cnt_trip,
                     d.esite};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -159,7 +156,26 @@ hipError_t ensure_nl(komd_ctx* c, bool dirs) {
     if (dirs && !d.nld) KOMD_CHECK_HIP(dalloc(&d.nld, (size_t)c->n * kFullCap));
     return hipSuccess;
 }
+}  // namespace
+
+namespace komd {
+komd_status api_fail(komd_ctx* c, komd_status s, const std::string& msg) { return fail(c, s, msg); }
+komd_status api_sync(komd_ctx* c, const char* where) { return sync_check(c, where); }
+// COM, wrap, force reset, binning, then the full neighbour lists (no pair force)
+hipError_t api_prepare_lists(komd_ctx* c) {
+    KOMD_CHECK_HIP(launch_com_wrap_key(c, false));
+    KOMD_CHECK_HIP(launch_bin(c));
+    KOMD_CHECK_HIP(ensure_nl(c, true));
+    return launch_cell(c, false, 2, false, false);
+}
+}  // namespace komd
+
+namespace {
+
 int atm_code(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind == KOMD_3B_ATM_REF ? 2 : 0); }
+// neighbour-list mode of a three-body kind: 0 none, 1 lists, 2 lists + cell directions
+// (the reference-mode ATM filter and the surrogate's per-cell force rows read them)
+int nl_mode(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind ? 2 : 0); }
 
 // Three-body forces on their own (ForceFunctor3B::operator(), ForceFunctor3B.cpp:10-14).
 hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
@@ -168,8 +184,12 @@ hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
         KOMD_CHECK_HIP(launch_cell(c, false, atm_code(kind), false, false));
         return launch_atm(c, atm_code(kind), energy, nullptr, nullptr, nullptr);
     }
-    if (kind == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, energy, 0);
-    if (kind == KOMD_3B_HYBRID) return hybrid_forces(c, energy);
+    if (kind == KOMD_3B_SURROGATE || kind == KOMD_3B_HYBRID) {
+        KOMD_CHECK_HIP(ensure_nl(c, true));
+        KOMD_CHECK_HIP(launch_cell(c, false, 2, false, false));
+        if (kind == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, energy, 0);
+        return hybrid_forces(c, energy);
+    }
     return hipErrorInvalidValue;
 }
 
@@ -180,7 +200,7 @@ hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
 hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     if (bonded) KOMD_CHECK_HIP(launch_bonded(c, false));
     const int atm = atm_code(kind3b);
-    const int nl = atm ? atm : (kind3b ? 1 : 0);
+    const int nl = nl_mode(kind3b);
     if (nl) KOMD_CHECK_HIP(ensure_nl(c, nl == 2));
     KOMD_CHECK_HIP(launch_cell(c, true, nl, false, true));
     if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr);
@@ -327,7 +347,7 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     double** f64[] = {&d.rx, &d.ry, &d.rz, &d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.eps, &d.sig, &d.esite};
     for (auto pp : f64)
         if (e == hipSuccess) e = dalloc(pp, (size_t)n);
-    uint32_t** u32n[] = {&d.idcls, &d.key, &d.slot, &d.tmp, &d.sorted, &d.sid};
+    uint32_t** u32n[] = {&d.idcls, &d.key, &d.slot, &d.tmp, &d.sorted, &d.inv, &d.sid};
     for (auto pp : u32n)
         if (e == hipSuccess) e = dalloc(pp, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.mol_start, mstart.size());
@@ -546,11 +566,11 @@ komd_status komd_compute_forces(komd_ctx* c, int32_t kind3b, int32_t use_bonded)
     return sync_check(c, "komd_compute_forces");
 }
 
-komd_status komd_run(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0) {
-    komd_status s = need_sites(c);
-    if (s) return s;
-    if (nsteps < 0) return fail(c, KOMD_ERR_ARG, "negative step count");
-    if (kind3b < 0 || kind3b > KOMD_3B_HYBRID) return fail(c, KOMD_ERR_ARG, "unknown 3-body kind");
+}  // extern "C"
+
+namespace {
+komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0) {
+    komd_status s = KOMD_OK;
     const bool bonded = use_bonded && (c->nbonds + c->nangles) > 0;
     const bool thermo = t0 > 0;
     if (thermo) {
@@ -559,6 +579,17 @@ komd_status komd_run(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bo
         HIPCALL(c, hipStreamSynchronize(c->stream), "komd_run");
     }
     c->run_t0 = t0;
+    {
+        // device buffers of the chosen passes exist before any graph capture (no allocation while capturing)
+        const int atm = atm_code(kind3b);
+        const int nl = nl_mode(kind3b);
+        if (nl) HIPCALL(c, ensure_nl(c, nl == 2), "komd_run");
+        if (kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID) {
+            hipError_t e = surrogate_prepare(c, kind3b == KOMD_3B_HYBRID);
+            if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+            HIPCALL(c, e, "komd_run");
+        }
+    }
     if (c->timing || nsteps < 3) {
         for (int64_t k = 0; k < nsteps; k++) {
             hipError_t e = enqueue_step(c, kind3b, bonded, thermo);
@@ -600,6 +631,69 @@ komd_status komd_run(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bo
     return sync_check(c, "komd_run");
 }
 
+}  // namespace
+
+extern "C" {
+
+komd_status komd_run(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (nsteps < 0) return fail(c, KOMD_ERR_ARG, "negative step count");
+    if (kind3b < 0 || kind3b > KOMD_3B_HYBRID) return fail(c, KOMD_ERR_ARG, "unknown 3-body kind");
+    const bool learned = kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID;
+    if (!learned || c->judge_every <= 0 || c->judge_fell) {
+        if (learned && c->judge_fell) {
+            // a judged run fell back: the exact ATM replaces the surrogate from here on
+            kind3b = KOMD_3B_ATM_EXACT;
+        }
+        return run_plain(c, nsteps, kind3b, use_bonded, t0);
+    }
+    // exact ATM as judge every judge_every steps; fallback when the error is too large
+    int64_t done = 0;
+    while (done < nsteps) {
+        const int64_t seg = std::min<int64_t>(c->judge_every, nsteps - done);
+        s = run_plain(c, seg, kind3b, use_bonded, t0);
+        if (s) return s;
+        done += seg;
+        double err[2];
+        s = komd_surrogate_judge(c, err);
+        if (s) return s;
+        c->judge_checks++;
+        c->judge_last = err[0];
+        c->judge_worst = std::max(c->judge_worst, err[0]);
+        if (err[0] > c->judge_max) {
+            c->judge_fell = 1;
+            kind3b = KOMD_3B_ATM_EXACT;
+            s = komd_compute_forces(c, kind3b, use_bonded);  // current forces from the exact ATM
+            if (s) return s;
+            if (done < nsteps) return run_plain(c, nsteps - done, kind3b, use_bonded, t0);
+            return KOMD_OK;
+        }
+    }
+    return KOMD_OK;
+}
+
+komd_status komd_set_judge(komd_ctx* c, int64_t every, double max_rel_rms) {
+    if (!c) return KOMD_ERR_ARG;
+    if (every < 0 || !(max_rel_rms > 0)) return fail(c, KOMD_ERR_ARG, "judge: every >= 0, max_rel_rms > 0");
+    c->judge_every = every;
+    c->judge_max = max_rel_rms;
+    c->judge_checks = 0;
+    c->judge_last = c->judge_worst = 0;
+    c->judge_fell = 0;
+    return KOMD_OK;
+}
+
+komd_status komd_judge_report(komd_ctx* c, int64_t* checks, double* last_rel_rms, double* worst_rel_rms,
+                              int32_t* fell_back) {
+    if (!c) return KOMD_ERR_ARG;
+    if (checks) *checks = c->judge_checks;
+    if (last_rel_rms) *last_rel_rms = c->judge_last;
+    if (worst_rel_rms) *worst_rel_rms = c->judge_worst;
+    if (fell_back) *fell_back = c->judge_fell;
+    return KOMD_OK;
+}
+
 komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* out) {
     komd_status s = need_sites(c);
     if (s) return s;
@@ -619,7 +713,7 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
     double* out_dev = d.scal + 8;
     double vals[8];
     const int atm = atm_code(kind3b);
-    const int nl = atm ? atm : (kind3b ? 1 : 0);
+    const int nl = nl_mode(kind3b);
     if (nl) HIPCALL(c, ensure_nl(c, nl == 2), "komd_measure_observables");
     HIPCALL(c, launch_cell(c, true, nl, true, false), "komd_measure_observables");
     HIPCALL(c, launch_reduce_parts(c, d.red, cell_blocks(c), 2, out_dev), "komd_measure_observables");
@@ -650,7 +744,7 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
         hipError_t e = forces_3b(c, kind3b, true);
         if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
         HIPCALL(c, e, "komd_measure_observables");
-        HIPCALL(c, launch_reduce_parts(c, d.red, c->red_blocks, 1, out_dev), "komd_measure_observables");
+        HIPCALL(c, launch_reduce_parts(c, d.red, surrogate_energy_blocks(c), 1, out_dev), "komd_measure_observables");
         s = sync_check(c, "komd_measure_observables");
         if (s) return s;
         HIPCALL(c, hipMemcpy(vals, out_dev, sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");

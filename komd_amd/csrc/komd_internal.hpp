@@ -7,6 +7,7 @@
 //   key[N] u32     domain flat cell of every site (x + nx (y + ny z), 0-based)
 //   slot[N] u32    arrival slot inside its cell (atomic histogram)
 //   count[C], start[C+1] u32, tmp[N], sorted[N] u32   counting sort, stable per cell
+//   inv[N] u32     cell-sorted slot of every site (inverse of sorted)
 //   spos[N] float4 cell-sorted FP32 position relative to the cell centre, .w = site index bits
 //   sprm[N] float2 cell-sorted (sigma, sqrt(eps)) when the sites are not uniform
 //   sid[N]  u32    cell-sorted id class when molecules have several sites
@@ -55,7 +56,7 @@ struct DevState {
     uint32_t* mol_start = nullptr;
     // binning
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr,
-             *sorted = nullptr;
+             *sorted = nullptr, *inv = nullptr;
     float4* spos = nullptr;
     float2* sprm = nullptr;
     uint32_t* sid = nullptr;
@@ -112,6 +113,12 @@ struct komd_ctx {
     double run_t0 = 0;         // thermostat target of the running komd_run
     int hyb_axis = 0;
     double hyb_center = 0, hyb_half = 0, hyb_blend = 0;
+    // exact-ATM judge of surrogate runs
+    int64_t judge_every = 0;
+    double judge_max = 0;
+    int64_t judge_checks = 0;
+    double judge_last = 0, judge_worst = 0;
+    int judge_fell = 0;
 };
 
 namespace komd {
@@ -153,8 +160,12 @@ int atm_blocks(komd_ctx* c);
 // k_bonded.hip
 hipError_t launch_bonded(komd_ctx* c, bool energy);
 hipError_t launch_limit(komd_ctx* c);
-// k_surrogate.hip
+// k_surrogate.hip.  Both expect current neighbour lists.  With energy, the three-body
+// energy partials are left in d.red, `three_body_energy_blocks` of them.
 hipError_t launch_surrogate_forces(komd_ctx* c, bool energy, int hybrid);
+hipError_t hybrid_forces(komd_ctx* c, bool energy);
+int surrogate_energy_blocks(komd_ctx* c);
+hipError_t surrogate_prepare(komd_ctx* c, bool hybrid);  // allocates the pass buffers (not capturable)
 void surrogate_free(komd_ctx* c);
 
 }  // namespace komd

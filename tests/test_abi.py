@@ -22,3 +22,12 @@ def test_wrapper_signatures_cover_the_header():
 def test_library_is_in_tree():
     assert os.path.dirname(K.LIB_PATH) == os.path.dirname(os.path.abspath(K.__file__))
     ctypes.CDLL(K.LIB_PATH)
+
+
+def test_committed_surrogate_weights():
+    """The shipped surrogate parameters have the documented size and are finite."""
+    import numpy as np
+    from komd_amd.train import DEFAULT_WEIGHTS, NPARAM
+    p = np.load(DEFAULT_WEIGHTS)
+    assert p.dtype == np.float32 and p.size == NPARAM and np.isfinite(p).all()
+    assert NPARAM == 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48

@@ -1,0 +1,192 @@
+"""GPU parity of the learned three-body surrogate against the FP64 oracle
+(oracle/surrogate.py), through the C ABI.
+
+Bars (stated per test):
+  * descriptors: FP32 sums of FP32 basis values -> relative 2e-4 of each column's scale;
+  * energies / forces / training gradient: BF16 MFMA operands with FP32 accumulation
+    -> RMS error a few 1e-3 of the quantity's spread, tolerances written in each test.
+A wrong MFMA operand layout or a wrong backward term shows up as O(1) errors.
+"""
+import numpy as np
+import pytest
+
+import komd_amd as K
+from komd_amd import systems as S
+from oracle import surrogate as SO
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.073
+
+
+def system(cells=5, seed=3, jitter=0.1):
+    return S.fcc(cells, 0.8, 0.8, seed, jitter=jitter)
+
+
+def oracle_inputs(s):
+    r = np.mod(s.r - s.low, s.box) + s.low
+    return SO.neighbour_pairs(r, np.asarray(s.box, np.float64))
+
+
+def params_for(s, seed=1):
+    i, j, d = oracle_inputs(s)
+    D, _ = SO.descriptors(s.n, i, d)
+    rng = np.random.default_rng(seed)
+    p = SO.unpack(SO.init_params(seed, shift=D.mean(0), scale=1.0 / (D.std(0) + 1e-9), e_mean=0.25, e_std=0.05))
+    p["b1"] = rng.normal(size=SO.H1) * 0.3
+    p["b2"] = rng.normal(size=SO.H2) * 0.3
+    return SO.pack(p).astype(np.float32)
+
+
+def test_descriptors_match_oracle():
+    s = system(5, 11)
+    print(f"descriptors n={s.n}", flush=True)
+    with K.Context(s) as c:
+        Dg = c.surrogate_descriptors()
+    i, j, d = oracle_inputs(s)
+    Do, _ = SO.descriptors(s.n, i, d)
+    scale = np.abs(Do).max(0) + 1e-12
+    err = np.abs(Dg - Do) / scale
+    print("max col-relative error", err.max())
+    assert err.max() < 2e-4
+
+
+def test_energies_match_oracle():
+    s = system(5, 12)
+    p = params_for(s, 2)
+    with K.Context(s) as c:
+        c.surrogate_set_params(p)
+        Eg = c.surrogate_energies()
+        assert np.array_equal(c.surrogate_get_params(), p)
+    i, j, d = oracle_inputs(s)
+    _, Eo, _ = SO.energy_forces(p.astype(np.float64), s.n, i, j, d)
+    spread = Eo.std()
+    rms = np.sqrt(np.mean((Eg - Eo) ** 2)) / spread
+    mx = np.abs(Eg - Eo).max() / spread
+    print(f"E rms {rms:.2e} max {mx:.2e} (of the spread {spread:.3e})")
+    # BF16 operands: relative rounding 2^-9 per operand, FP32 accumulation
+    assert rms < 2e-2 and mx < 1e-1
+
+
+@pytest.mark.parametrize("cells,seed", [(5, 13), (8, 14)])
+def test_forces_match_oracle(cells, seed):
+    s = system(cells, seed)
+    p = params_for(s, 3)
+    print(f"surrogate forces n={s.n}", flush=True)
+    with K.Context(s) as c:
+        c.surrogate_set_params(p)
+        c.update_container()
+        c.forces_3b(K.SURROGATE)
+        Fg = c.forces()
+        obs = c.observables(K.SURROGATE)
+    i, j, d = oracle_inputs(s)
+    Eo, Ei, Fo = SO.energy_forces(p.astype(np.float64), s.n, i, j, d)
+    scale = np.sqrt(np.mean(Fo ** 2))
+    dd = np.linalg.norm(Fg - Fo, axis=1)
+    rms = np.sqrt(np.mean(dd ** 2)) / scale
+    print(f"F rms {rms:.2e} max {dd.max() / scale:.2e}; sum F {np.abs(Fg.sum(0)).max():.2e}")
+    assert rms < 2e-2 and dd.max() / scale < 2e-1
+    assert np.abs(Fg.sum(0)).max() < 1e-3 * scale * np.sqrt(s.n)
+    assert abs(obs["u_3b"] - Eo) < 1e-2 * abs(Eo)
+
+
+def test_training_gradient_matches_oracle():
+    s = system(5, 15)
+    p = params_for(s, 4)
+    wE, wF = 40.0, 10.0
+    with K.Context(s, nu=NU) as c:
+        c.surrogate_set_params(p)
+        c.train_add_snapshot()
+        assert c.train_size() == 1
+        # the stored targets: exact ATM per-site energies and forces
+        E_ref = c.atm_site_energies()
+        c.update_container()
+        c.forces_3b(K.ATM_EXACT)
+        F_ref = c.forces()
+        loss_g, grad_g = c.train_gradient(0, wE, wF)
+        r_after, _, _ = c.get_state()
+    i, j, d = oracle_inputs(s)
+    loss_o, grad_o, _, _ = SO.loss_and_grad(p.astype(np.float64), s.n, i, j, d, E_ref, F_ref, wE, wF)
+    print(f"loss gpu {loss_g:.6e} oracle {loss_o:.6e}")
+    assert abs(loss_g - loss_o) < 2e-2 * loss_o
+    sl = slice(0, SO.NPARAM - 2 * SO.NDESC)
+    gg, go = grad_g[sl].astype(np.float64), grad_o[sl]
+    rel = np.linalg.norm(gg - go) / np.linalg.norm(go)
+    cos = gg @ go / (np.linalg.norm(gg) * np.linalg.norm(go))
+    print(f"gradient relative error {rel:.3e}, cosine {cos:.6f}")
+    assert rel < 5e-2 and cos > 0.998
+    # each block of the gradient separately (a missing term in one block shows here)
+    o = 0
+    for name, k in (("W1", 64 * 48), ("b1", 64), ("W2", 64 * 64), ("b2", 64), ("w3", 64), ("b3", 1)):
+        bg, bo = gg[o:o + k], go[o:o + k]
+        rb = np.linalg.norm(bg - bo) / (np.linalg.norm(bo) + 1e-30)
+        print(f"  {name}: relative error {rb:.3e}")
+        assert rb < 1e-1, name
+        o += k
+    assert not np.array_equal(r_after, None)
+
+
+def test_training_reduces_loss():
+    s = system(5, 16)
+    with K.Context(s, nu=NU, dt=0.005) as c:
+        for k in range(6):
+            c.run(10, kind3b=K.ATM_EXACT, t0=0.8)
+            c.train_add_snapshot()
+        c.train_standardize()
+        e_mean, e_std, f_rms = c.train_stats()
+        p = SO.unpack(c.surrogate_get_params().astype(np.float64))
+        p0 = SO.unpack(SO.init_params(5, shift=p["shift"], scale=p["scale"], e_mean=e_mean, e_std=e_std))
+        c.surrogate_set_params(SO.pack(p0).astype(np.float32))
+        wE, wF = 0.1 / e_std ** 2, 1.0 / f_rms ** 2
+        losses = [c.train_epoch(2e-3, wE, wF, seed=k) for k in range(25)]
+        print("losses", [f"{x:.3e}" for x in losses[::4]])
+        assert losses[-1] < 0.5 * losses[0]
+        rel, mx = c.surrogate_judge()
+        print(f"judge after training: rel rms {rel:.3f} max {mx:.3f}")
+        assert rel < 1.0
+
+
+def test_judge_and_fallback():
+    s = system(5, 17)
+    p = params_for(s, 6)
+    with K.Context(s, nu=NU, dt=0.005) as c:
+        c.surrogate_set_params(p)
+        rel, mx = c.surrogate_judge()
+        assert rel > 0 and mx >= rel
+        c.compute_forces(K.SURROGATE)
+        c.set_judge(2, 1e-9)          # any error triggers the fallback
+        c.run(6, kind3b=K.SURROGATE)
+        rep = c.judge_report()
+        assert rep["checks"] == 1 and rep["fell_back"]
+        # after the fallback the forces are the exact ATM ones
+        r, v, f = c.get_state()
+    with K.Context(s, nu=NU, dt=0.005) as e:
+        e.set_state(r, v)
+        e.compute_forces(K.ATM_EXACT)
+        fe = e.forces()
+    # same configuration, same kernels; only the FP32 summation order of the pair pass may differ
+    assert np.abs(f - fe).max() < 1e-5 * np.abs(fe).max()
+
+
+def test_trained_surrogate_gain():
+    """What the trained network gains over the methods without it: three-body force error
+    against the exact ATM on an equilibrated argon state (rho 0.8, T* 0.8), for the
+    committed weights, the mean-field ATM2B scaling (ATM2B.cpp:9-20) and LJ alone."""
+    from komd_amd.train import DEFAULT_WEIGHTS
+    s = S.fcc(10, 0.8, 0.8, 2024)
+    with K.Context(s, nu=NU, dt=0.005) as c:
+        c.surrogate_set_params(np.load(DEFAULT_WEIGHTS))
+        c.compute_forces(K.ATM_EXACT)
+        c.run(400, kind3b=K.ATM_EXACT)
+        rel, mx = c.surrogate_judge()
+        c.update_container()
+        c.forces_3b(K.ATM_EXACT)
+        f_atm = c.forces()
+        c.update_container()
+        c.forces_pair()
+        f_lj = c.forces()
+    scale = np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1)))
+    err_2b = np.sqrt(np.mean(np.sum((-(2 * NU * 0.8 / 3) * f_lj - f_atm) ** 2, axis=1))) / scale
+    print(f"three-body force error: surrogate {rel:.4f} (max {mx:.3f}), ATM2B {err_2b:.4f}, LJ only 1.0")
+    assert rel < 0.06
+    assert rel < 0.25 * err_2b
