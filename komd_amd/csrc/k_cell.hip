@@ -344,6 +344,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     const int p = lb * kAtmWaves + wvu;
     double e_acc = 0.0;
     unsigned long long nvisit = 0;
+    bool p_skip = false;
     if (p < a.n) {
         const int nn = __builtin_amdgcn_readfirstlane((int)a.nlc[p]);
         const uint32_t oi = a.sorted[p];
@@ -351,140 +352,149 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
         if (HYB) li = a.lam[oi];
         const float4* row = a.nl + (size_t)p * kFullCap;
         float4 xa = make_float4(1.0e6f * (float)(lane + 1), 0.f, 0.f, 0.f);  // distinct far-away filler
+        bool weighted = HYB ? li != 0.f : true;
         for (int s = lane; s < nn; s += 64) {
             const float4 v = row[s];
             wbuf[wvu][s] = v;
             if (s == lane) xa = v;
             if (ATM == 2) wdir[wvu][s] = a.nld[(size_t)p * kFullCap + s];
-            if (HYB) wlam[wvu][s] = a.lam[__float_as_uint(v.w) & kIdxMask];
+            if (HYB) {
+                const float lj = a.lam[__float_as_uint(v.w) & kIdxMask];
+                wlam[wvu][s] = lj;
+                weighted = weighted || lj != 0.f;
+            }
         }
         wave_sync();
-        uint32_t* ring = qbuf[wvu];
-        float fx = 0.f, fy = 0.f, fz = 0.f, ev = 0.f;
-        // dense evaluation of an in-range pair (x = r_j - r_i, y = r_k - r_i)
-        auto force = [&](int ia, int ib) {
-            const float4 xv = wbuf[wvu][ia], yv = wbuf[wvu][ib];
-            const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
-            float wt = 1.f;
-            if (HYB) wt = (li + wlam[wvu][ia] + wlam[wvu][ib]) * (1.f / 3.f);
-            const float X = xv.x * xv.x + xv.y * xv.y + xv.z * xv.z;
-            const float Y = yv.x * yv.x + yv.y * yv.y + yv.z * yv.z;
-            const float Z = zx * zx + zy * zy + zz * zz;
-            const float p1 = xv.x * yv.x + xv.y * yv.y + xv.z * yv.z;
-            const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
-            const float p3 = yv.x * zx + yv.y * zy + yv.z * zz;
-            const float D = X * Y * Z;
-            const float Pp = -(p1 * p2 * p3);
-            const float rD = __builtin_amdgcn_rsqf(D);
-            const float rD2 = rD * rD;
-            const float D52 = rD2 * rD2 * rD;
-            float K = 3.f * a.nu * D52 * wt;
-            if (ATM == 2) K = -K;  // reference: +grad V
-            const float A = -(D + 5.f * Pp);
-            const float iX = __builtin_amdgcn_rcpf(X), iY = __builtin_amdgcn_rcpf(Y);
-            const float t1 = p2 * p3, t23 = p1 * (p3 + p2);
-            fx += K * (A * (xv.x * iX + yv.x * iY) - t1 * (xv.x + yv.x) - t23 * zx);
-            fy += K * (A * (xv.y * iX + yv.y * iY) - t1 * (xv.y + yv.y) - t23 * zy);
-            fz += K * (A * (xv.z * iX + yv.z * iY) - t1 * (xv.z + yv.z) - t23 * zz);
-            ev += a.nu * D52 * (D + 3.f * Pp);  // unweighted V of this visit
-        };
-        const unsigned long long below = (1ull << lane) - 1ull;
-        int head = 0, tail = 0;
-        // ballot-compact this instruction's in-range pairs into the ring; dense flush at 64
-        auto push = [&](bool hit, int ia, int ib) {
-            const unsigned long long bal = __ballot(hit);
-            if (hit) ring[(tail + __popcll(bal & below)) & 127] = (uint32_t)ia | ((uint32_t)ib << 16);
-            tail = __builtin_amdgcn_readfirstlane(tail + (int)__popcll(bal));
-            if (tail - head >= 64) {
-                wave_sync();
-                const uint32_t e = ring[(head + lane) & 127];
-                force((int)(e & 0xFFFFu), (int)(e >> 16));
-                head += 64;
-                wave_sync();
-            }
-        };
-        auto accept = [&](bool in, int ia, int ib) {
-            if (ATM == 2 && in) {
-                const int da = wdir[wvu][ia], db = wdir[wvu][ib];
-                const bool same = (da == 13) && (db == 13);
-                const bool distinct = (da != 13) && (db != 13) && (da != db);
-                in = same || distinct;
-            }
-            if (HYB && in) in = (li + wlam[wvu][ia] + wlam[wvu][ib]) != 0.f;
-            return in;
-        };
-        if (nn <= 64) {
-            // filter in registers: lane a holds x_a; after k DPP rotations (wave_rol)
-            // lane a holds y = x_{a-k mod 64} and its lane index yi
-            float yx = xa.x, yy = xa.y, yz = xa.z;
-            int yi = lane;
-            for (int k = 1; k <= 32; k++) {
-                yx = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yx), 0x134, 0xF, 0xF, false));
-                yy = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yy), 0x134, 0xF, 0xF, false));
-                yz = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yz), 0x134, 0xF, 0xF, false));
-                yi = __builtin_amdgcn_update_dpp(0, yi, 0x134, 0xF, 0xF, false);
-                const float zx = yx - xa.x, zy = yy - xa.y, zz = yz - xa.z;
+        // hybrid: a site whose neighbourhood lies wholly outside the exact region has no weighted triplet
+        if (HYB && __ballot(weighted) == 0ull) p_skip = true;
+        if (!p_skip) {
+            uint32_t* ring = qbuf[wvu];
+            float fx = 0.f, fy = 0.f, fz = 0.f, ev = 0.f;
+            // dense evaluation of an in-range pair (x = r_j - r_i, y = r_k - r_i)
+            auto force = [&](int ia, int ib) {
+                const float4 xv = wbuf[wvu][ia], yv = wbuf[wvu][ib];
+                const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
+                float wt = 1.f;
+                if (HYB) wt = (li + wlam[wvu][ia] + wlam[wvu][ib]) * (1.f / 3.f);
+                const float X = xv.x * xv.x + xv.y * xv.y + xv.z * xv.z;
+                const float Y = yv.x * yv.x + yv.y * yv.y + yv.z * yv.z;
                 const float Z = zx * zx + zy * zy + zz * zz;
-                const bool valid = lane < nn && yi < nn && (k < 32 || lane < 32);
-                bool in = false;
-                if (valid) {
-                    if (fabsf(Z - a.rc2f) <= a.band3)
-                        in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xa.w),
-                                        __float_as_uint(wbuf[wvu][yi].w)) > a.rc2);
-                    else
-                        in = Z <= a.rc2f;
-                    in = accept(in, lane, yi);
+                const float p1 = xv.x * yv.x + xv.y * yv.y + xv.z * yv.z;
+                const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
+                const float p3 = yv.x * zx + yv.y * zy + yv.z * zz;
+                const float D = X * Y * Z;
+                const float Pp = -(p1 * p2 * p3);
+                const float rD = __builtin_amdgcn_rsqf(D);
+                const float rD2 = rD * rD;
+                const float D52 = rD2 * rD2 * rD;
+                float K = 3.f * a.nu * D52 * wt;
+                if (ATM == 2) K = -K;  // reference: +grad V
+                const float A = -(D + 5.f * Pp);
+                const float iX = __builtin_amdgcn_rcpf(X), iY = __builtin_amdgcn_rcpf(Y);
+                const float t1 = p2 * p3, t23 = p1 * (p3 + p2);
+                fx += K * (A * (xv.x * iX + yv.x * iY) - t1 * (xv.x + yv.x) - t23 * zx);
+                fy += K * (A * (xv.y * iX + yv.y * iY) - t1 * (xv.y + yv.y) - t23 * zy);
+                fz += K * (A * (xv.z * iX + yv.z * iY) - t1 * (xv.z + yv.z) - t23 * zz);
+                ev += a.nu * D52 * (D + 3.f * Pp);  // unweighted V of this visit
+            };
+            const unsigned long long below = (1ull << lane) - 1ull;
+            int head = 0, tail = 0;
+            // ballot-compact this instruction's in-range pairs into the ring; dense flush at 64
+            auto push = [&](bool hit, int ia, int ib) {
+                const unsigned long long bal = __ballot(hit);
+                if (hit) ring[(tail + __popcll(bal & below)) & 127] = (uint32_t)ia | ((uint32_t)ib << 16);
+                tail = __builtin_amdgcn_readfirstlane(tail + (int)__popcll(bal));
+                if (tail - head >= 64) {
+                    wave_sync();
+                    const uint32_t e = ring[(head + lane) & 127];
+                    force((int)(e & 0xFFFFu), (int)(e >> 16));
+                    head += 64;
+                    wave_sync();
                 }
-                push(in, lane, yi);
-            }
-        } else {
-            // long lists: circulant pairs (a, a+s+1 mod n) for s < H = (n-1)/2, plus
-            // s = H for a < n/2 when n is even, read from LDS
-            const int H = (nn - 1) >> 1;
-            const int halfn = (nn & 1) ? 0 : (nn >> 1);
-            for (int sft = 0; sft <= H; sft++) {
-                const int lim = sft < H ? nn : halfn;
-                for (int a0 = 0; a0 < lim; a0 += 64) {
-                    const int ia = a0 + lane;
+            };
+            auto accept = [&](bool in, int ia, int ib) {
+                if (ATM == 2 && in) {
+                    const int da = wdir[wvu][ia], db = wdir[wvu][ib];
+                    const bool same = (da == 13) && (db == 13);
+                    const bool distinct = (da != 13) && (db != 13) && (da != db);
+                    in = same || distinct;
+                }
+                if (HYB && in) in = (li + wlam[wvu][ia] + wlam[wvu][ib]) != 0.f;
+                return in;
+            };
+            if (nn <= 64) {
+                // filter in registers: lane a holds x_a; after k DPP rotations (wave_rol)
+                // lane a holds y = x_{a-k mod 64} and its lane index yi
+                float yx = xa.x, yy = xa.y, yz = xa.z;
+                int yi = lane;
+                for (int k = 1; k <= 32; k++) {
+                    yx = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yx), 0x134, 0xF, 0xF, false));
+                    yy = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yy), 0x134, 0xF, 0xF, false));
+                    yz = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yz), 0x134, 0xF, 0xF, false));
+                    yi = __builtin_amdgcn_update_dpp(0, yi, 0x134, 0xF, 0xF, false);
+                    const float zx = yx - xa.x, zy = yy - xa.y, zz = yz - xa.z;
+                    const float Z = zx * zx + zy * zy + zz * zz;
+                    const bool valid = lane < nn && yi < nn && (k < 32 || lane < 32);
                     bool in = false;
-                    int ib = 0;
-                    if (ia < lim) {
-                        const float4 xv = wbuf[wvu][ia];
-                        ib = ia + sft + 1;
-                        if (ib >= nn) ib -= nn;
-                        const float4 yv = wbuf[wvu][ib];
-                        const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
-                        const float Z = zx * zx + zy * zy + zz * zz;
+                    if (valid) {
                         if (fabsf(Z - a.rc2f) <= a.band3)
-                            in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xv.w), __float_as_uint(yv.w)) >
-                                   a.rc2);
+                            in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xa.w),
+                                            __float_as_uint(wbuf[wvu][yi].w)) > a.rc2);
                         else
                             in = Z <= a.rc2f;
-                        in = accept(in, ia, ib);
+                        in = accept(in, lane, yi);
                     }
-                    push(in, ia, ib);
+                    push(in, lane, yi);
+                }
+            } else {
+                // long lists: circulant pairs (a, a+s+1 mod n) for s < H = (n-1)/2, plus
+                // s = H for a < n/2 when n is even, read from LDS
+                const int H = (nn - 1) >> 1;
+                const int halfn = (nn & 1) ? 0 : (nn >> 1);
+                for (int sft = 0; sft <= H; sft++) {
+                    const int lim = sft < H ? nn : halfn;
+                    for (int a0 = 0; a0 < lim; a0 += 64) {
+                        const int ia = a0 + lane;
+                        bool in = false;
+                        int ib = 0;
+                        if (ia < lim) {
+                            const float4 xv = wbuf[wvu][ia];
+                            ib = ia + sft + 1;
+                            if (ib >= nn) ib -= nn;
+                            const float4 yv = wbuf[wvu][ib];
+                            const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
+                            const float Z = zx * zx + zy * zy + zz * zz;
+                            if (fabsf(Z - a.rc2f) <= a.band3)
+                                in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xv.w), __float_as_uint(yv.w)) >
+                                       a.rc2);
+                            else
+                                in = Z <= a.rc2f;
+                            in = accept(in, ia, ib);
+                        }
+                        push(in, ia, ib);
+                    }
                 }
             }
-        }
-        wave_sync();
-        if (lane < tail - head) {
-            const uint32_t e = ring[(head + lane) & 127];
-            force((int)(e & 0xFFFFu), (int)(e >> 16));
-        }
-        nvisit = (unsigned long long)tail;
-        double dx = wave_sum((double)fx), dy = wave_sum((double)fy), dz = wave_sum((double)fz);
-        const double de = wave_sum((double)ev) * (1.0 / 3.0);  // this site's third of its triplets
-        if (lane == 0) {
-            if (HYB) {  // -V grad(w): site i receives -(sum V/3) dl_i/dx_axis
-                const double t = -de * (double)a.dlam[oi];
-                if (a.hyb_axis == 0) dx += t; else if (a.hyb_axis == 1) dy += t; else dz += t;
+            wave_sync();
+            if (lane < tail - head) {
+                const uint32_t e = ring[(head + lane) & 127];
+                force((int)(e & 0xFFFFu), (int)(e >> 16));
             }
-            a.fx[oi] += dx;
-            a.fy[oi] += dy;
-            a.fz[oi] += dz;
-            if (a.esite) a.esite[oi] = HYB ? de * (double)li : de;
+            nvisit = (unsigned long long)tail;
+            double dx = wave_sum((double)fx), dy = wave_sum((double)fy), dz = wave_sum((double)fz);
+            const double de = wave_sum((double)ev) * (1.0 / 3.0);  // this site's third of its triplets
+            if (lane == 0) {
+                if (HYB) {  // -V grad(w): site i receives -(sum V/3) dl_i/dx_axis
+                    const double t = -de * (double)a.dlam[oi];
+                    if (a.hyb_axis == 0) dx += t; else if (a.hyb_axis == 1) dy += t; else dz += t;
+                }
+                a.fx[oi] += dx;
+                a.fy[oi] += dy;
+                a.fz[oi] += dz;
+                if (a.esite) a.esite[oi] = HYB ? de * (double)li : de;
+            }
+            e_acc = HYB ? de * (double)li : de;
         }
-        e_acc = HYB ? de * (double)li : de;
     }
     // triplet visits: one slot per workgroup, accumulated across launches
     if (lane == 0) {

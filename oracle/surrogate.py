@@ -315,3 +315,103 @@ def atm2b_scale(nu, rho, eps=1.0, sig=1.0):
     """LJ scaling factor 1 - 2 nu rho / (3 eps sig^6) of the reference's ATM2B potential
     (potentials/ATM2B.cpp:9-20); its three-body force estimate is (scale - 1) * F_LJ."""
     return 1.0 - 2.0 * nu * rho / (3.0 * eps * sig ** 6)
+
+
+# ---- hybrid: exact ATM in a slab, surrogate outside, cos^2 blend -----------------------------
+def lam_of(x, axis_len, center, half, blend):
+    """lambda = 1 for |x - c| <= h (minimum image), cos^2 descent to 0 over blend, 0 beyond;
+    returns lambda and d lambda / dx."""
+    d = x - center
+    d = d - axis_len * np.rint(d / axis_len)
+    ad = np.abs(d)
+    lam = np.where(ad <= half, 1.0, 0.0)
+    dl = np.zeros_like(x)
+    mid = (ad > half) & (ad < half + blend)
+    t = 0.5 * np.pi * (ad[mid] - half) / blend
+    lam[mid] = np.cos(t) ** 2
+    dl[mid] = -2.0 * np.cos(t) * np.sin(t) * (0.5 * np.pi / blend) * np.sign(d[mid])
+    return lam, dl
+
+
+def atm_triplets(r, box, rc=RC):
+    """Every triplet i < j < k with all three minimum-image distances below rc."""
+    i, j, d = neighbour_pairs(r, box, rc)
+    nb = [[] for _ in range(len(r))]
+    for a, b in zip(i, j):
+        nb[a].append(b)
+    out = []
+    for a in range(len(r)):
+        js = np.array(sorted(x for x in nb[a] if x > a), dtype=np.int64)
+        if len(js) < 2:
+            continue
+        jj, kk = np.meshgrid(js, js, indexing="ij")
+        sel = jj < kk
+        jj, kk = jj[sel], kk[sel]
+        dv = r[kk] - r[jj]
+        dv -= box * np.rint(dv / box)
+        ok = (dv * dv).sum(1) < rc * rc
+        out.append(np.stack([np.full(ok.sum(), a), jj[ok], kk[ok]], 1))
+    return np.concatenate(out) if out else np.zeros((0, 3), np.int64)
+
+
+def atm_triplet_terms(r, box, trip, nu):
+    """V per triplet and dV/dr of each member (physical ATM), FP64 closed form:
+    V = nu (1 + 3 cos1 cos2 cos3) / (r_ij r_ik r_jk)^3."""
+    ri, rj, rk = r[trip[:, 0]], r[trip[:, 1]], r[trip[:, 2]]
+    a = rj - ri
+    a -= box * np.rint(a / box)
+    b = rk - ri
+    b -= box * np.rint(b / box)
+    c = b - a
+    A, B, C = (a * a).sum(1), (b * b).sum(1), (c * c).sum(1)
+    p1, p2, p3 = (a * b).sum(1), (a * c).sum(1), (b * c).sum(1)
+    Dd = A * B * C
+    P = -p1 * p2 * p3
+    V = nu * Dd ** -2.5 * (Dd + 3 * P)
+
+    # gradient of V(a, b) with c = b - a, by the chain rule on (A, B, C, p1, p2, p3)
+    dV_dD = nu * (-1.5 * Dd ** -2.5 - 7.5 * P * Dd ** -3.5)
+    dV_dP = 3 * nu * Dd ** -2.5
+    dD_dA, dD_dB, dD_dC = B * C, A * C, A * B
+    dP_dp1, dP_dp2, dP_dp3 = -p2 * p3, -p1 * p3, -p1 * p2
+    gA, gB, gC = dV_dD * dD_dA, dV_dD * dD_dB, dV_dD * dD_dC
+    g1, g2, g3 = dV_dP * dP_dp1, dV_dP * dP_dp2, dV_dP * dP_dp3
+    # dV/da and dV/db (a, b independent; c = b - a)
+    dVa = (2 * gA[:, None] * a + g1[:, None] * b + g2[:, None] * (c - a) - g3[:, None] * b
+           - 2 * gC[:, None] * c)
+    dVb = (2 * gB[:, None] * b + g1[:, None] * a + g2[:, None] * a + g3[:, None] * (c + b)
+           + 2 * gC[:, None] * c)
+    # r_i enters as -(a + b), r_j as a, r_k as b
+    return V, -(dVa + dVb), dVa, dVb
+
+
+def hybrid_energy_forces(p, r, box, nu, axis, center, half, blend):
+    """E = sum_t w_t V_t + sum_i (1 - lambda_i) E_i^sur, w_t = mean lambda of the members;
+    F = -dE/dr including the gradients of the weights."""
+    r = np.asarray(r, np.float64)
+    n = len(r)
+    lam, dl = lam_of(r[:, axis], box[axis], center, half, blend)
+    F = np.zeros((n, 3))
+    # exact part
+    trip = atm_triplets(r, box)
+    E = 0.0
+    if len(trip):
+        V, gi, gj, gk = atm_triplet_terms(r, box, trip, nu)
+        w = lam[trip].mean(1)
+        E += float((w * V).sum())
+        for col, g in ((0, gi), (1, gj), (2, gk)):
+            np.add.at(F, trip[:, col], -w[:, None] * g)
+            np.add.at(F[:, axis], trip[:, col], -V * dl[trip[:, col]] / 3.0)
+    # surrogate part: site energies weighted by 1 - lambda
+    cw = 1.0 - lam
+    i, j, d = neighbour_pairs(r, box)
+    D, T = descriptors(n, i, d)
+    m = mlp(p, D)
+    G = m["G"] * cw[:, None]
+    A = _state(G, T)
+    c_own = _pair_grad(G[:, :NR], A, i, d)
+    c_nbr = _pair_grad(G[:, :NR], A, j, -d)
+    np.add.at(F, i, c_own - c_nbr)
+    F[:, axis] += m["E"] * dl
+    E += float((cw * m["E"]).sum())
+    return E, F

@@ -190,3 +190,53 @@ def test_trained_surrogate_gain():
     print(f"three-body force error: surrogate {rel:.4f} (max {mx:.3f}), ATM2B {err_2b:.4f}, LJ only 1.0")
     assert rel < 0.06
     assert rel < 0.25 * err_2b
+
+
+@pytest.mark.parametrize("half,blend", [(1.5, 2.0), (0.8, 1.0)])
+def test_hybrid_matches_oracle(half, blend):
+    """Exact ATM weighted by the mean lambda of each triplet plus surrogate site energies
+    weighted by 1 - lambda, forces including the weight gradients (config-5 blend)."""
+    s = system(5, 18, jitter=0.1)
+    p = params_for(s, 7)
+    box = np.asarray(s.box, np.float64)
+    center = s.low[0] + 0.5 * box[0]
+    with K.Context(s, nu=NU) as c:
+        c.surrogate_set_params(p)
+        c.set_hybrid(0, center, half, blend)
+        c.update_container()
+        c.forces_3b(K.HYBRID)
+        Fg = c.forces()
+        obs = c.observables(K.HYBRID)
+        r_w, _, _ = c.get_state()
+    E_o, F_o = SO.hybrid_energy_forces(p.astype(np.float64), r_w, box, NU, 0, center, half, blend)
+    scale = np.sqrt(np.mean(np.sum(F_o ** 2, axis=1)))
+    dd = np.linalg.norm(Fg - F_o, axis=1)
+    rms = np.sqrt(np.mean(dd ** 2)) / scale
+    print(f"hybrid F rms {rms:.2e} max {dd.max() / scale:.2e}; E gpu {obs['u_3b']:.6f} oracle {E_o:.6f}")
+    # ATM part FP32 (1e-5), surrogate part BF16 operands
+    assert rms < 2e-2 and dd.max() / scale < 2e-1
+    assert abs(obs["u_3b"] - E_o) < 1e-2 * abs(E_o)
+
+
+def test_hybrid_limits():
+    """lambda = 1 everywhere reproduces the exact ATM; lambda = 0 everywhere the surrogate."""
+    s = system(5, 19)
+    p = params_for(s, 8)
+    with K.Context(s, nu=NU) as c:
+        c.surrogate_set_params(p)
+        c.update_container()
+        c.forces_3b(K.ATM_EXACT)
+        f_atm = c.forces()
+        c.update_container()
+        c.forces_3b(K.SURROGATE)
+        f_sur = c.forces()
+        c.set_hybrid(1, 0.0, 1e3, 1.0)          # every site inside the exact region
+        c.update_container()
+        c.forces_3b(K.HYBRID)
+        f1 = c.forces()
+        c.set_hybrid(1, s.low[1] + 0.123, 0.0, 1e-4)  # no site within 1e-4 of the plane
+        c.update_container()
+        c.forces_3b(K.HYBRID)
+        f0 = c.forces()
+    assert np.abs(f1 - f_atm).max() <= 1e-6 * np.abs(f_atm).max()
+    assert np.abs(f0 - f_sur).max() <= 1e-5 * np.abs(f_sur).max()

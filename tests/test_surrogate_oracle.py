@@ -126,3 +126,43 @@ def test_atm2b_scale():
     # ATM2B.cpp:9-20 at rho = 0.8, nu = 0.073, eps = sigma = 1
     assert SO.atm2b_scale(0.073, 0.8) == pytest.approx(1 - 2 * 0.073 * 0.8 / 3, abs=1e-15)
     assert SO.NPARAM == 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48
+
+
+def test_atm_triplet_terms_against_closed_forms():
+    # equilateral triangle of side s: V = 11 nu / (8 s^9); gradient by central differences
+    s_, nu = 1.1, 0.073
+    r = np.array([[1.0, 1.0, 1.0], [1.0 + s_, 1.0, 1.0], [1.0 + s_ / 2, 1.0 + s_ * np.sqrt(3) / 2, 1.0]])
+    box = np.array([20.0, 20.0, 20.0])
+    trip = np.array([[0, 1, 2]])
+    V, gi, gj, gk = SO.atm_triplet_terms(r, box, trip, nu)
+    assert V[0] == pytest.approx(11 * nu / (8 * s_ ** 9), rel=1e-12)
+    rng = np.random.default_rng(2)
+    r = r + rng.normal(size=r.shape) * 0.1
+    V, gi, gj, gk = SO.atm_triplet_terms(r, box, trip, nu)
+    h = 1e-6
+    for m, g in enumerate((gi, gj, gk)):
+        for ax in range(3):
+            rp = r.copy(); rp[m, ax] += h
+            rm = r.copy(); rm[m, ax] -= h
+            fd = (SO.atm_triplet_terms(rp, box, trip, nu)[0] - SO.atm_triplet_terms(rm, box, trip, nu)[0]) / (2 * h)
+            assert fd[0] == pytest.approx(g[0, ax], rel=1e-6, abs=1e-10)
+
+
+def test_hybrid_forces_are_minus_gradient():
+    r, box = small_system(3, 9)
+    n = len(r)
+    p = random_params(5)
+    args = (0.073, 0, box[0] * 0.5, 0.8, 1.2)   # nu, axis, center, half width, blend
+    E, F = SO.hybrid_energy_forces(p, r, box, *args)
+    lam, _ = SO.lam_of(r[:, 0], box[0], box[0] * 0.5, 0.8, 1.2)
+    assert (lam > 0).any() and (lam < 1).any() and ((lam > 0) & (lam < 1)).any()
+    rng = np.random.default_rng(4)
+    h = 1e-5
+    # sites in the blend region carry the weight-gradient terms
+    blend_sites = np.nonzero((lam > 0.05) & (lam < 0.95))[0]
+    for s in list(rng.choice(blend_sites, min(3, len(blend_sites)), replace=False)) + list(rng.choice(n, 2)):
+        for ax in range(3):
+            rp = r.copy(); rp[s, ax] += h
+            rm = r.copy(); rm[s, ax] -= h
+            fd = -(SO.hybrid_energy_forces(p, rp, box, *args)[0] - SO.hybrid_energy_forces(p, rm, box, *args)[0]) / (2 * h)
+            assert abs(fd - F[s, ax]) < 2e-6 * max(1.0, abs(F[s, ax])), (s, ax, fd, F[s, ax])
