@@ -1,4 +1,5 @@
-"""Builds libkomd_hip.so (gfx950) in-tree with hipcc.
+"""Builds libkomd_hip.so (gfx950) in-tree with hipcc, and the C++ host driver komd_md
+(komd_amd/host, g++) linked against it.
 
 Usage: python -m komd_amd.build   (or komd_amd.build.build() from Python)
 """
@@ -15,6 +16,9 @@ OUT = os.path.join(HERE, "libkomd_hip.so")
 OBJ = os.path.join(HERE, "_build")
 
 SOURCES = ["komd_api.hip", "k_container.hip", "k_cell.hip", "k_bonded.hip", "k_surrogate.hip"]
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["komd_host.cpp", "main.cpp"]
+HOST_EXE = os.path.join(HERE, "komd_md")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KOMD_ARCH", "gfx950")
@@ -47,9 +51,22 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, OUT)
+    build_host()
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {HOST_EXE}")
     return OUT
+
+
+def build_host() -> str:
+    """g++ build of the host driver; it finds libkomd_hip.so next to itself (rpath $ORIGIN)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", *[os.path.join(HOST, s) for s in HOST_SOURCES],
+           "-o", HOST_EXE + ".tmp", "-L" + HERE, "-l:libkomd_hip.so", "-Wl,-rpath,$ORIGIN",
+           "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host build failed:\n{res.stderr}")
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
+    return HOST_EXE
 
 
 if __name__ == "__main__":

@@ -1,0 +1,315 @@
+// C++ host layer above libkomd_hip.so (see komd_host.hpp).
+#include "komd_host.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <sstream>
+
+namespace komd_host {
+
+void check(komd_ctx* ctx, komd_status st, const char* what) {
+    if (st == KOMD_OK) return;
+    std::string msg = std::string(what) + ": ";
+    msg += ctx ? komd_last_error(ctx) : "invalid argument";
+    throw std::runtime_error(msg);
+}
+
+// ---------------------------------------------------------------------------
+// deck (IO/FileInput.cpp:11-121): whitespace-separated "key values" pairs, '#' starts a
+// comment line; keys not used by the force path are read past.
+bool readFile(const std::string& path, Configuration& config, std::vector<Component>& components, std::string* err) {
+    auto fail = [&](const std::string& m) {
+        if (err) *err = m;
+        return false;
+    };
+    std::ifstream file(path);
+    if (!file.is_open()) return fail("Could not open the file: " + path);
+    double max_eps = std::numeric_limits<double>::min(), max_sig = std::numeric_limits<double>::min();
+    std::string var;
+    auto d3in = [&](d3& x) { file >> x[0] >> x[1] >> x[2]; };
+    while (file >> var) {
+        if (var[0] == '#') {
+            file.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+            continue;
+        }
+        if (var == "temperature") file >> config.temperature;
+        else if (var == "delta_t") file >> config.delta_t;
+        else if (var == "cutoff") file >> config.cutoff;
+        else if (var == "cell_size") file >> config.cell_size;
+        else if (var == "limit") file >> config.limit_factor;
+        else if (var == "energy_3b") file >> config.energy_3b;
+        else if (var == "domain_low") d3in(config.domainLow);
+        else if (var == "domain_high") d3in(config.domainHigh);
+        else if (var == "timesteps") file >> config.timesteps;
+        else if (var == "write_freq") file >> config.write_freq;
+        else if (var == "enable_3b") file >> config.enable_3b;
+        else if (var == "enable_3b_direct") file >> config.enable_3b_direct;
+        else if (var == "store_checkpoint") file >> config.storeCheckpoint;
+        else if (var == "checkpoint_file") {
+            std::string p;
+            d3 off;
+            file >> p;
+            d3in(off);
+            config.checkpoint_files.emplace_back(p, off);
+        } else if (var == "COMP") {
+            int cid;
+            Component::Site s;
+            file >> cid >> s.eps >> s.sig >> s.mass >> s.r[0] >> s.r[1] >> s.r[2];
+            if (cid < 0) return fail("negative component id not allowed");
+            if ((int)components.size() <= cid) components.resize((size_t)cid + 1);
+            components[(size_t)cid].sites.push_back(s);
+            max_eps = std::max(max_eps, s.eps);
+            max_sig = std::max(max_sig, s.sig);
+        }
+        // GPU path keys
+        else if (var == "atm_mode") file >> config.atm_mode;
+        else if (var == "surrogate_weights") file >> config.surrogate_weights;
+        else if (var == "judge_every") file >> config.judge_every;
+        else if (var == "judge_max") file >> config.judge_max;
+        else if (var == "hybrid_region") file >> config.hybrid_axis >> config.hybrid_center >> config.hybrid_half >> config.hybrid_blend;
+        else if (var == "thermostat") file >> config.thermostat;
+        else if (var == "checkpoint_precision") file >> config.checkpoint_precision;
+        if (file.fail()) return fail("malformed value for key " + var);
+    }
+    if (components.empty()) return fail("no components defined");
+    if (max_eps == std::numeric_limits<double>::min()) return fail("invalid epsilon specified");
+    if (max_sig == std::numeric_limits<double>::min()) return fail("invalid sigma specified");
+    if (config.checkpoint_files.empty()) return fail("No phasespace data specified");
+    if (config.cell_size < config.cutoff) return fail("cell size must be larger or equal to cutoff radius");
+    if (config.atm_mode != "reference" && config.atm_mode != "exact") return fail("atm_mode: reference or exact");
+    return true;
+}
+
+void Sites::add(uint64_t mol_id, const d3& pos, const d3& vel, double m, double e, double s) {
+    r.insert(r.end(), pos.begin(), pos.end());
+    v.insert(v.end(), vel.begin(), vel.end());
+    f.insert(f.end(), 3, 0.0);
+    mass.push_back(m);
+    eps.push_back(e);
+    sig.push_back(s);
+    id.push_back(mol_id);
+}
+
+// ---------------------------------------------------------------------------
+// checkpoints (IO/CheckpointIO.cpp:13-76): "id  rx ry rz  vx vy vz  mass eps sig" per
+// site; consecutive lines with one id form one molecule; epsilon is internal.
+namespace CheckpointIO {
+
+void readPs(const std::string& path, const d3& offset, Sites& sites) {
+    std::ifstream file(path);
+    if (!file.is_open()) throw std::runtime_error("Could not open checkpoint file: " + path);
+    std::string line;
+    size_t lineno = 0;
+    while (std::getline(file, line)) {
+        lineno++;
+        std::istringstream in(line);
+        uint64_t id;
+        d3 r, v;
+        double m, e, s;
+        if (!(in >> id)) continue;  // blank line
+        if (!(in >> r[0] >> r[1] >> r[2] >> v[0] >> v[1] >> v[2] >> m >> e >> s))
+            throw std::runtime_error(path + ":" + std::to_string(lineno) + ": malformed site line");
+        for (int k = 0; k < 3; k++) r[k] += offset[k];
+        sites.add(id, r, v, m, e, s);
+    }
+}
+
+void loadCheckpoint(const Configuration& config, Sites& sites) {
+    for (const auto& cf : config.checkpoint_files) readPs(cf.first, cf.second, sites);
+}
+
+void writePs(const std::string& path, const Sites& s, int precision) {
+    std::ofstream file(path);
+    if (!file.is_open()) throw std::runtime_error("Could not write checkpoint file: " + path);
+    file << std::setprecision(precision);
+    for (size_t i = 0; i < s.size(); i++) {
+        if (i) file << "\n";
+        file << s.id[i] << "\t" << s.r[3 * i] << " " << s.r[3 * i + 1] << " " << s.r[3 * i + 2] << "\t" << s.v[3 * i]
+             << " " << s.v[3 * i + 1] << " " << s.v[3 * i + 2] << "\t" << s.mass[i] << " " << s.eps[i] << " "
+             << s.sig[i];
+    }
+}
+
+void writeCheckpoint(uint64_t simstep, MoleculeContainer& container, int precision, const std::string& path) {
+    container.writeSOA2AOS();
+    writePs(path.empty() ? "ckpt_" + std::to_string(simstep) + ".ps" : path, container.sites(), precision);
+}
+
+}  // namespace CheckpointIO
+
+// ---------------------------------------------------------------------------
+MoleculeContainer::~MoleculeContainer() {
+    if (m_ctx) komd_destroy(m_ctx);
+}
+
+void MoleculeContainer::constructSOAs() {
+    const size_t n = m_sites.size();
+    check(m_ctx, komd_set_sites(m_ctx, (int64_t)n, m_sites.r.data(), m_sites.v.data(), m_sites.mass.data(),
+                                m_sites.eps.data(), m_sites.sig.data(), m_sites.id.data()),
+          "constructSOAs");
+    m_num_molecules = 0;
+    for (size_t i = 0; i < n; i++)
+        if (i == 0 || m_sites.id[i] != m_sites.id[i - 1]) m_num_molecules++;
+}
+
+void MoleculeContainer::writeSOA2AOS() {
+    check(m_ctx, komd_get_sites(m_ctx, m_sites.r.data(), m_sites.v.data(), m_sites.f.data()), "writeSOA2AOS");
+}
+
+LinkedCells::LinkedCells(const Configuration& config) {
+    for (int k = 0; k < 3; k++) {
+        m_box.low[k] = config.domainLow[k];
+        m_box.high[k] = config.domainHigh[k];
+    }
+    m_box.cell_size = config.cell_size;
+    m_params.cutoff = config.cutoff;
+    m_params.delta_t = config.delta_t;
+    m_params.limit_factor = config.limit_factor;
+    m_params.nu = config.energy_3b * 1e9 * kConvJEi;  // ATM.cpp:13
+    m_params.temp_factor = kTemperatureFactor;
+    m_params.bin_mode = KOMD_BIN_COM;
+}
+
+void LinkedCells::init() {
+    komd_ctx* c = nullptr;
+    const komd_status st = komd_create(&m_box, &m_params, &c);
+    m_ctx = c;
+    check(m_ctx, st, "LinkedCells::init");
+    constructSOAs();
+}
+
+void LinkedCells::updateContainer() { check(m_ctx, komd_update_container(m_ctx), "updateContainer"); }
+
+std::array<int64_t, 3> LinkedCells::gridDims() const {
+    std::array<int64_t, 3> d{};
+    check(m_ctx, komd_grid_dims(m_ctx, d.data()), "gridDims");
+    return d;
+}
+
+void LJ12_6::handlePairList() { check(p_container.ctx(), komd_forces_pair(p_container.ctx(), KOMD_PAIR_LJ12_6), "LJ12_6"); }
+void Limit::handlePairList() { check(p_container.ctx(), komd_forces_limit(p_container.ctx()), "Limit"); }
+void HarmonicBonds::handlePairList() { check(p_container.ctx(), komd_forces_bonded(p_container.ctx()), "HarmonicBonds"); }
+void ATM::handleTripleList() { check(p_container.ctx(), komd_forces_3b(p_container.ctx(), m_kind), "ATM"); }
+
+// parameters: a .npy file of KOMD_SUR_NPARAM little-endian float32 values (1-D)
+static std::vector<float> read_params(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) throw std::runtime_error("Could not open surrogate weights: " + path);
+    std::vector<char> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    size_t off = 0;
+    if (buf.size() >= 10 && std::memcmp(buf.data(), "\x93NUMPY", 6) == 0) {
+        const int major = (unsigned char)buf[6];
+        size_t hl = 0;
+        if (major == 1) {
+            hl = (unsigned char)buf[8] | ((size_t)(unsigned char)buf[9] << 8);
+            off = 10 + hl;
+        } else {
+            hl = (unsigned char)buf[8] | ((size_t)(unsigned char)buf[9] << 8) | ((size_t)(unsigned char)buf[10] << 16) |
+                 ((size_t)(unsigned char)buf[11] << 24);
+            off = 12 + hl;
+        }
+        const std::string header(buf.data() + (major == 1 ? 10 : 12), hl);
+        if (header.find("'<f4'") == std::string::npos || header.find("'fortran_order': False") == std::string::npos)
+            throw std::runtime_error(path + ": expected a C-ordered float32 array");
+    }
+    const size_t bytes = buf.size() - off;
+    if (bytes != (size_t)KOMD_SUR_NPARAM * 4)
+        throw std::runtime_error(path + ": expected " + std::to_string(KOMD_SUR_NPARAM) + " float32 values");
+    std::vector<float> p(KOMD_SUR_NPARAM);
+    std::memcpy(p.data(), buf.data() + off, bytes);
+    return p;
+}
+
+ATMSurrogate::ATMSurrogate(MoleculeContainer& container, const std::string& weights, bool hybrid)
+    : ForceFunctor3B(container), m_kind(hybrid ? KOMD_3B_HYBRID : KOMD_3B_SURROGATE) {
+    const std::vector<float> p = read_params(weights);
+    check(container.ctx(), komd_surrogate_set_params(container.ctx(), p.data(), (int64_t)p.size()), "ATMSurrogate");
+}
+void ATMSurrogate::handleTripleList() {
+    check(p_container.ctx(), komd_forces_3b(p_container.ctx(), m_kind), "ATMSurrogate");
+}
+
+void Integrator::integrate0() { check(p_container.ctx(), komd_integrate0(p_container.ctx()), "integrate0"); }
+void Integrator::integrate1() { check(p_container.ctx(), komd_integrate1(p_container.ctx()), "integrate1"); }
+
+void TemperatureSensor::measure() {
+    check(m_container.ctx(), komd_measure_temperature(m_container.ctx(), &m_temperature), "TemperatureSensor");
+}
+
+void VelocityScaling::apply() { check(m_container.ctx(), komd_rescale(m_container.ctx(), m_target), "VelocityScaling"); }
+
+std::unique_ptr<Registry> makeRegistry(const Configuration& config) {
+    auto reg = std::make_unique<Registry>();
+    reg->config = config;
+    auto lc = std::make_unique<LinkedCells>(config);
+    CheckpointIO::loadCheckpoint(config, lc->sites());
+    lc->init();
+    reg->container = std::move(lc);
+    MoleculeContainer& c = *reg->container;
+    reg->forceFunctors.push_back(std::make_shared<LJ12_6>(c));
+    reg->forceFunctors.push_back(std::make_shared<Limit>(c));
+    if (config.enable_3b) {
+        c.enable3B();
+        if (!config.surrogate_weights.empty()) {
+            const bool hybrid = config.hybrid_axis >= 0;
+            if (hybrid)
+                check(c.ctx(), komd_set_hybrid(c.ctx(), config.hybrid_axis, config.hybrid_center, config.hybrid_half,
+                                               config.hybrid_blend),
+                      "hybrid_region");
+            reg->forceFunctors3b.push_back(std::make_shared<ATMSurrogate>(c, config.surrogate_weights, hybrid));
+            if (config.judge_every > 0)
+                check(c.ctx(), komd_set_judge(c.ctx(), config.judge_every, config.judge_max), "judge_every");
+        } else {
+            reg->forceFunctors3b.push_back(std::make_shared<ATM>(c, config.atm_mode == "reference"));
+        }
+    }
+    reg->integrators.push_back(std::make_unique<Integrator>(c));
+    reg->temperatureSensor = std::make_unique<TemperatureSensor>(c);
+    if (config.thermostat) reg->thermostats.push_back(std::make_unique<VelocityScaling>(c, config.temperature));
+    return reg;
+}
+
+void Simulation::run(Registry& reg, uint64_t steps) {
+    MoleculeContainer& c = *reg.container;
+    // initial pass to compute forces
+    c.updateContainer();
+    for (auto& p : reg.forceFunctors) (*p)();
+    if (c.threeBody())
+        for (auto& p : reg.forceFunctors3b) (*p)();
+    reg.temperatureSensor->measure();
+    for (auto& t : reg.thermostats) t->apply();
+    // main loop
+    for (uint64_t s = 0; s < steps; s++) {
+        for (auto& i : reg.integrators) i->integrate0();
+        c.updateContainer();
+        for (auto& p : reg.forceFunctors) (*p)();
+        if (c.threeBody())
+            for (auto& p : reg.forceFunctors3b) (*p)();
+        for (auto& i : reg.integrators) i->integrate1();
+        reg.temperatureSensor->measure();
+        m_simstep++;
+        for (auto& t : reg.thermostats) t->apply();
+    }
+}
+
+void Simulation::runDevice(Registry& reg, uint64_t steps) {
+    MoleculeContainer& c = *reg.container;
+    int kind = 0;
+    if (c.threeBody() && !reg.forceFunctors3b.empty()) {
+        if (auto a = std::dynamic_pointer_cast<ATM>(reg.forceFunctors3b[0])) kind = a->kind();
+        if (auto s = std::dynamic_pointer_cast<ATMSurrogate>(reg.forceFunctors3b[0])) kind = s->kind();
+    }
+    const double t0 = reg.thermostats.empty() ? 0.0 : reg.config.temperature;
+    check(c.ctx(), komd_compute_forces(c.ctx(), kind, 0), "Simulation::runDevice");
+    if (t0 > 0) {
+        reg.temperatureSensor->measure();
+        for (auto& t : reg.thermostats) t->apply();
+    }
+    check(c.ctx(), komd_run(c.ctx(), (int64_t)steps, kind, 0, t0), "Simulation::runDevice");
+    m_simstep += steps;
+}
+
+}  // namespace komd_host

@@ -1,0 +1,270 @@
+// C++ host layer above libkomd_hip.so: the simulation objects of a KoMD run (container,
+// force functors, integrator, temperature sensor, thermostat, simulation loop, deck and
+// checkpoint I/O) with the reference's names and call order, each one calling the C ABI
+// (include/komd.h).  Errors surface as std::runtime_error carrying komd_last_error(),
+// as the reference reports failures by exceptions and error logs.
+//
+// Reference interfaces mirrored (KoMD src/):
+//   ForceFunctor / ForceFunctor3B   potentials/ForceFunctor.h:15-33, ForceFunctor3B.h:14-34
+//   MoleculeContainer / LinkedCells  container/MoleculeContainer.h:19-146, LinkedCells.cpp:11-81
+//   Integrator                       integrators/Integrator.h:14-65
+//   TemperatureSensor                sensors/TemperatureSensor.cpp:22-43
+//   Thermostat / VelocityScaling     thermostats/VelocityScaling.cpp:12-25
+//   Simulation::run                  Simulation.cpp:10-80
+//   FileInput::readFile              IO/FileInput.cpp:11-121
+//   CheckpointIO                     IO/CheckpointIO.cpp:13-76
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/komd.h"
+
+namespace komd_host {
+
+using d3 = std::array<double, 3>;
+
+// Unit conversions of the reference (util/constants.h:12-20, TemperatureSensor.cpp:39-42):
+// epsilon and energy_3b are read in J and J nm^9; the internal energy unit is u A^2/ps^2.
+constexpr double kConvJEi = 6.022140754e22;
+// T = factor * sum(m v.v) / (3 N), factor = (A/ps in m/s)^2 * (Da in kg) / k_B
+constexpr double kTemperatureFactor = 1e4 * (1.66053906892e-27 / 1.380649e-23);
+
+// Deck keys used by the force path (Configuration.h, defaults.h), plus the keys of the
+// GPU path's own three-body options.
+struct Configuration {
+    double temperature = 300;
+    double delta_t = 0.001;
+    uint64_t timesteps = 1;
+    double cutoff = 2.5 * 3.41;
+    double cell_size = 3.0 * 3.41;
+    double limit_factor = 20;
+    double energy_3b = 1;
+    d3 domainLow{0, 0, 0};
+    d3 domainHigh{10, 10, 10};
+    uint64_t write_freq = 10;
+    bool storeCheckpoint = false;
+    bool enable_3b = false;
+    bool enable_3b_direct = true;
+    std::vector<std::pair<std::string, d3>> checkpoint_files;
+    // GPU path options (no reference counterpart)
+    std::string atm_mode = "reference";  // "reference": ATM_NOLIST semantics; "exact": physical, all triplets
+    std::string surrogate_weights;       // three-body surrogate parameters (.f32, KOMD_SUR_NPARAM floats)
+    int64_t judge_every = 0;             // exact-ATM judge interval of surrogate runs
+    double judge_max = 0.05;             // fallback threshold (relative RMS force error)
+    int hybrid_axis = -1;                // >= 0: hybrid exact/surrogate region along this axis
+    double hybrid_center = 0, hybrid_half = 0, hybrid_blend = 2.0;
+    bool thermostat = true;              // velocity rescaling every step (main.cpp:93)
+    int checkpoint_precision = 17;       // digits of the .ps writer
+};
+
+struct Component {  // one COMP line per site: eps (J), sigma, mass, offset
+    struct Site {
+        double eps, sig, mass;
+        d3 r;
+    };
+    std::vector<Site> sites;
+};
+
+// FileInput::readFile: false (with the message) on an unreadable deck or a failed check.
+bool readFile(const std::string& path, Configuration& config, std::vector<Component>& components, std::string* err);
+
+// Host copy of the sites in the reference's layout (SOA.h:11-38): interleaved d3 r and v,
+// per-site mass / epsilon (internal units) / sigma, molecule id per site.
+struct Sites {
+    std::vector<double> r, v, f, mass, eps, sig;
+    std::vector<uint64_t> id;
+    size_t size() const { return mass.size(); }
+    void add(uint64_t mol_id, const d3& pos, const d3& vel, double m, double e, double s);
+};
+
+class MoleculeContainer;
+
+namespace CheckpointIO {
+// loadCheckpoint: every checkpoint_file of the deck, positions shifted by its offset.
+void loadCheckpoint(const Configuration& config, Sites& sites);
+void readPs(const std::string& path, const d3& offset, Sites& sites);
+// writeCheckpoint: "ckpt_<simstep>.ps" (or `path`), one site per line, `precision` digits.
+void writeCheckpoint(uint64_t simstep, MoleculeContainer& container, int precision, const std::string& path = "");
+void writePs(const std::string& path, const Sites& sites, int precision);
+}  // namespace CheckpointIO
+
+// MoleculeContainer: owns the device context.
+class MoleculeContainer {
+public:
+    virtual ~MoleculeContainer();
+    virtual void init() = 0;
+    virtual void updateContainer() = 0;
+    uint64_t getNumMolecules() const { return m_num_molecules; }
+    // constructSOAs: host sites -> device; writeSOA2AOS: device -> host sites
+    void constructSOAs();
+    void writeSOA2AOS();
+    Sites& sites() { return m_sites; }
+    komd_ctx* ctx() const { return m_ctx; }
+    void enable3B() { m_enable_3b = true; }
+    void disable3B() { m_enable_3b = false; }
+    bool threeBody() const { return m_enable_3b; }
+
+protected:
+    Sites m_sites;
+    komd_ctx* m_ctx = nullptr;
+    uint64_t m_num_molecules = 0;
+    bool m_enable_3b = false;
+};
+
+class LinkedCells : public MoleculeContainer {
+public:
+    explicit LinkedCells(const Configuration& config);
+    void init() override;
+    void updateContainer() override;
+    std::array<int64_t, 3> gridDims() const;
+
+private:
+    komd_box m_box{};
+    komd_params m_params{};
+};
+
+// ForceFunctor: operator() runs the pass and waits for it (ForceFunctor.cpp:10-14).
+class ForceFunctor {
+public:
+    explicit ForceFunctor(MoleculeContainer& container) : p_container(container) {}
+    virtual ~ForceFunctor() = default;
+    void operator()() { handlePairList(); }
+
+protected:
+    virtual void handlePairList() = 0;
+    MoleculeContainer& p_container;
+};
+
+class LJ12_6 : public ForceFunctor {
+public:
+    using ForceFunctor::ForceFunctor;
+
+protected:
+    void handlePairList() override;
+};
+
+class Limit : public ForceFunctor {
+public:
+    using ForceFunctor::ForceFunctor;
+
+protected:
+    void handlePairList() override;
+};
+
+class HarmonicBonds : public ForceFunctor {  // new physics in FENE's functor slot
+public:
+    using ForceFunctor::ForceFunctor;
+
+protected:
+    void handlePairList() override;
+};
+
+class ForceFunctor3B {
+public:
+    explicit ForceFunctor3B(MoleculeContainer& container) : p_container(container) {}
+    virtual ~ForceFunctor3B() = default;
+    void operator()() { handleTripleList(); }
+
+protected:
+    virtual void handleTripleList() = 0;
+    MoleculeContainer& p_container;
+};
+
+// ATM_NOLIST (reference semantics) and ATM (physical, complete) share one functor.
+class ATM : public ForceFunctor3B {
+public:
+    ATM(MoleculeContainer& container, bool reference_semantics)
+        : ForceFunctor3B(container), m_kind(reference_semantics ? KOMD_3B_ATM_REF : KOMD_3B_ATM_EXACT) {}
+    int kind() const { return m_kind; }
+
+protected:
+    void handleTripleList() override;
+    int m_kind;
+};
+
+class ATMSurrogate : public ForceFunctor3B {
+public:
+    ATMSurrogate(MoleculeContainer& container, const std::string& weights, bool hybrid);
+    int kind() const { return m_kind; }
+
+protected:
+    void handleTripleList() override;
+    int m_kind;
+};
+
+class Integrator {
+public:
+    explicit Integrator(MoleculeContainer& container) : p_container(container) {}
+    virtual ~Integrator() = default;
+    virtual void integrate0();
+    virtual void integrate1();
+
+protected:
+    MoleculeContainer& p_container;
+};
+
+class TemperatureSensor {
+public:
+    explicit TemperatureSensor(MoleculeContainer& container) : m_container(container) {}
+    void measure();
+    double getTemperature() const { return m_temperature; }
+
+private:
+    MoleculeContainer& m_container;
+    double m_temperature = 0;
+};
+
+class Thermostat {
+public:
+    virtual ~Thermostat() = default;
+    virtual void apply() = 0;
+};
+
+class VelocityScaling : public Thermostat {
+public:
+    VelocityScaling(MoleculeContainer& container, double target) : m_container(container), m_target(target) {}
+    void apply() override;
+
+private:
+    MoleculeContainer& m_container;
+    double m_target;
+};
+
+// The objects of one run, filled in the order main.cpp:48-93 registers them.
+struct Registry {
+    Configuration config;
+    std::unique_ptr<MoleculeContainer> container;
+    std::vector<std::shared_ptr<ForceFunctor>> forceFunctors;
+    std::vector<std::shared_ptr<ForceFunctor3B>> forceFunctors3b;
+    std::vector<std::unique_ptr<Integrator>> integrators;
+    std::vector<std::unique_ptr<Thermostat>> thermostats;
+    std::unique_ptr<TemperatureSensor> temperatureSensor;
+};
+
+// Builds the registry of a deck: container from the checkpoint files, LJ (+ harmonic
+// bonds when a topology is set later) + Limit, the three-body functor of the deck, the
+// integrator, temperature sensor and (optionally) the rescaling thermostat.
+std::unique_ptr<Registry> makeRegistry(const Configuration& config);
+
+class Simulation {
+public:
+    // The reference loop (Simulation.cpp:49-74), one functor call after the other.
+    void run(Registry& reg, uint64_t steps);
+    // The same loop queued on the device as one replayed step (komd_run).
+    void runDevice(Registry& reg, uint64_t steps);
+    uint64_t simstep() const { return m_simstep; }
+
+private:
+    uint64_t m_simstep = 0;
+};
+
+// status check of a C-ABI call
+void check(komd_ctx* ctx, komd_status st, const char* what);
+
+}  // namespace komd_host

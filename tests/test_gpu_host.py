@@ -1,0 +1,120 @@
+"""The C++ host layer on the GPU: the reference's per-functor loop (Simulation.cpp:49-74)
+driven through komd_md against the FP64 oracle loop, the device-queued loop against the
+per-functor loop, and the reference's own example checkpoint through the pair kernel."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import komd_amd as K
+from komd_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "komd_amd", "komd_md")
+IBI_PS = os.path.join(ROOT, "tests", "golden", "ref_ibi_ckpt_100k_2b_equil.ps")
+NU = 0.073
+# reduced units in the reference's deck units (sigma = epsilon = m = 1, tau = 1 ps)
+T_K = 0.8 / 0.8314462609869346          # T* = 0.8
+E3B = NU / (1e9 * 6.022140754e22)         # nu* = 0.073
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle
+    oracle.build()
+    return oracle
+
+
+def g(x):
+    return f"{float(x):.17g}"
+
+
+def write_ps(path, s):
+    with open(path, "w") as fh:
+        for i in range(s.n):
+            fh.write(f"{int(s.mol_id[i])}\t{g(s.r[i,0])} {g(s.r[i,1])} {g(s.r[i,2])}\t{g(s.v[i,0])} {g(s.v[i,1])} "
+                     f"{g(s.v[i,2])}\t{g(s.mass[i])} {g(s.eps[i])} {g(s.sig[i])}\n")
+
+
+def deck_for(tmp_path, s, steps, extra=""):
+    ps = str(tmp_path / "start.ps")
+    write_ps(ps, s)
+    body = (f"temperature {g(T_K)}\ndelta_t 0.005\ncutoff 2.5\ncell_size {g(s.cell_size)}\ntimesteps {steps}\n"
+            f"checkpoint_file {ps} 0 0 0\nlimit 1e+12\nenergy_3b {g(E3B)}\n"
+            f"domain_low {g(s.low[0])} {g(s.low[1])} {g(s.low[2])}\n"
+            f"domain_high {g(s.high[0])} {g(s.high[1])} {g(s.high[2])}\n"
+            f"enable_3b 1\nCOMP 0 1.6605390688282807e-23 1 1 0 0 0\n{extra}")
+    p = tmp_path / "run.dat"
+    p.write_text(body)
+    return str(p)
+
+
+def run_md(*args):
+    print("komd_md", " ".join(args), flush=True)
+    res = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def test_reference_loop_matches_oracle(orc, tmp_path):
+    s = S.fcc(6, 0.8, 0.8, 71, jitter=0.05)
+    d = deck_for(tmp_path, s, 10)
+    out_ps = str(tmp_path / "end.ps")
+    info = run_md(d, "--write", out_ps)
+    assert info["steps"] == 10 and info["sites"] == s.n
+    got = np.loadtxt(out_ps)
+    r0, b = orc.update_container(s)
+    f0 = orc.lj_c08(s, r0, b) + orc.atm_c08(s, r0, b, NU, sign=1.0)[0]
+    # the host's first thermostat call rescales the initial velocities before the loop
+    T = orc.kinetic_sum(s, r0, s.v)
+    s2 = s.copy()
+    s2.v = s.v * np.sqrt(0.8 / (T[0] / (3 * T[1])))
+    r_o, v_o, _ = orc.run(s2, 10, dt=0.005, nu=NU, atm=1, atm_sign=1.0, T0=0.8, temp_factor=1.0, nthreads=8,
+                          f=f0, r=r0)
+    assert np.abs(got[:, 1:4] - r_o).max() < 1e-6
+    assert np.abs(got[:, 4:7] - v_o).max() < 1e-5
+    assert info["temperature"] == pytest.approx(T_K, rel=1e-8)
+
+
+def test_device_loop_matches_reference_loop(tmp_path):
+    s = S.fcc(6, 0.8, 0.8, 72, jitter=0.05)
+    d = deck_for(tmp_path, s, 12)
+    a, b = str(tmp_path / "a.ps"), str(tmp_path / "b.ps")
+    ia = run_md(d, "--write", a)
+    ib = run_md(d, "--device-loop", "--write", b)
+    ra, rb = np.loadtxt(a), np.loadtxt(b)
+    assert ia["loop"] == "reference" and ib["loop"] == "device"
+    # same kernels, different queueing: FP32 summation order of the pair pass only
+    assert np.abs(ra[:, 1:4] - rb[:, 1:4]).max() < 1e-9
+    assert np.abs(ra[:, 4:7] - rb[:, 4:7]).max() < 1e-7
+
+
+def test_surrogate_deck(tmp_path):
+    s = S.fcc(6, 0.8, 0.8, 73, jitter=0.05)
+    w = os.path.join(ROOT, "komd_amd", "weights", "atm_surrogate.npy")
+    d = deck_for(tmp_path, s, 20, extra=f"surrogate_weights {w}\njudge_every 10\njudge_max 0.5\n")
+    info = run_md(d, "--device-loop")
+    assert info["steps"] == 20 and np.isfinite(info["u_3b"]) and info["u_3b"] > 0
+
+
+def test_reference_checkpoint_lj_forces(orc):
+    """examples/IBI/ckpt_100k_2b_equil.ps (343 sites, sigma 1.5, internal epsilon 0.6,
+    mass 4) in its deck's box (15^3, cutoff 5): LJ forces against the oracle."""
+    a = np.loadtxt(IBI_PS)
+    s = S.System(r=a[:, 1:4].copy(), v=a[:, 4:7].copy(), mass=a[:, 7].copy(), eps=a[:, 8].copy(),
+                 sig=a[:, 9].copy(), mol_id=a[:, 0].astype(np.uint64), low=np.zeros(3), high=np.full(3, 15.0),
+                 cell_size=5.0, name="ibi_ckpt")
+    r, b = orc.update_container(s, )
+    f_or = orc.lj_c08(s, r, b, cutoff=5.0)
+    with K.Context(s, cutoff=5.0) as c:
+        c.update_container()
+        c.forces_pair()
+        f = c.forces()
+    scale = np.sqrt(np.mean(f_or ** 2))
+    dd = np.linalg.norm(f - f_or, axis=1)
+    print(f"reference checkpoint LJ: max {dd.max() / scale:.2e} rms {np.sqrt(np.mean(dd ** 2)) / scale:.2e}")
+    assert dd.max() / scale < 2e-4 and np.sqrt(np.mean(dd ** 2)) / scale < 5e-6

@@ -1,0 +1,95 @@
+"""C++ host layer (komd_amd/host, driver komd_md) on the CPU: deck parsing with the
+reference's keys and checks (IO/FileInput.cpp:11-121) and the .ps checkpoint reader /
+writer (IO/CheckpointIO.cpp:13-76), on the reference's own example checkpoint
+(examples/IBI/ckpt_100k_2b_equil.ps, copied as a data fixture)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "komd_amd", "komd_md")
+GOLD = os.path.join(ROOT, "tests", "golden")
+IBI_PS = os.path.join(GOLD, "ref_ibi_ckpt_100k_2b_equil.ps")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def driver():
+    if not os.path.exists(EXE):
+        from komd_amd import build
+        build.build_host()
+    return EXE
+
+
+def deck(tmp_path, body, name="run.dat"):
+    p = tmp_path / name
+    p.write_text(body)
+    return str(p)
+
+
+IBI_DECK = """# reference-style deck (examples/inp.dat keys)
+temperature 100
+delta_t 0.0005
+cutoff 5.0
+cell_size 5.0
+timesteps 7
+checkpoint_file {ps}    0 0 0
+limit 1e+9
+energy_3b 10
+domain_low 0 0 0
+domain_high 15 15 15
+enable_3b 1
+COMP 0 1.0e-21 1.5 4.0 0 0 0
+"""
+
+
+def run(*args, check=True):
+    res = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=60)
+    if check:
+        assert res.returncode == 0, res.stderr
+    return res
+
+
+def test_deck_and_reference_checkpoint(tmp_path):
+    d = deck(tmp_path, IBI_DECK.format(ps=IBI_PS))
+    out = json.loads(run(d, "--check").stdout)
+    ref = np.loadtxt(IBI_PS)
+    assert out["sites"] == len(ref) == 343 and out["molecules"] == 343 and out["components"] == 1
+    assert out["first_site"] == list(ref[0, 1:4])
+    assert out["cutoff"] == 5.0 and out["cell_size"] == 5.0 and out["timesteps"] == 7
+    assert out["domain_high"] == [15.0, 15.0, 15.0] and out["enable_3b"] == 1
+    # ATM.cpp:13: nu = energy_3b * 1e9 * conv_J_Ei
+    assert out["nu_internal"] == pytest.approx(10 * 1e9 * 6.022140754e22, rel=1e-15)
+
+
+def test_checkpoint_round_trip(tmp_path):
+    d = deck(tmp_path, IBI_DECK.format(ps=IBI_PS))
+    w = str(tmp_path / "out.ps")
+    run(d, "--check", "--write", w)
+    a, b = np.loadtxt(IBI_PS), np.loadtxt(w)
+    assert np.array_equal(a, b)
+    # the written file is itself a valid checkpoint with an offset applied on load
+    d2 = deck(tmp_path, IBI_DECK.format(ps=w).replace(f"{w}    0 0 0", f"{w}    1.5 0 -2"), "run2.dat")
+    out = json.loads(run(d2, "--check").stdout)
+    assert out["first_site"] == [a[0, 1] + 1.5, a[0, 2], a[0, 3] - 2]
+
+
+def test_reference_test_ps_multisite(tmp_path):
+    # examples/test/test.ps: four single-site molecules with ids 1..4
+    d = deck(tmp_path, IBI_DECK.format(ps=os.path.join(GOLD, "ref_test_ps.txt")))
+    out = json.loads(run(d, "--check").stdout)
+    assert out["sites"] == 4 and out["molecules"] == 4
+
+
+@pytest.mark.parametrize("mutation,message", [
+    (lambda s: s.replace("COMP 0 1.0e-21 1.5 4.0 0 0 0\n", ""), "no components defined"),
+    (lambda s: s.replace("cell_size 5.0", "cell_size 4.0"), "cell size must be larger or equal to cutoff radius"),
+    (lambda s: "\n".join(l for l in s.splitlines() if not l.startswith("checkpoint_file")), "No phasespace data"),
+    (lambda s: s.replace("COMP 0", "COMP -1"), "negative component id"),
+])
+def test_deck_errors(tmp_path, mutation, message):
+    d = deck(tmp_path, mutation(IBI_DECK.format(ps=IBI_PS)))
+    res = run(d, "--check", check=False)
+    assert res.returncode == 1 and message in res.stderr
