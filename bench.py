@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--drift-steps", type=int, default=10000, help="NVE steps for the energy drift (0: skip)")
     ap.add_argument("--learned-atoms", default="32000,256000",
                     help="comma list of system sizes for the learned leg ('' to skip)")
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the 1,048,576-atom hybrid slab leg")
+    ap.add_argument("--hybrid-steps", type=int, default=100)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -144,6 +146,76 @@ def learned_leg(n_atoms, args):
             ctx.set_state(r0, v0)
             ctx.compute_forces(K.SURROGATE)
             out["energy_drift_surrogate"] = drift_of(ctx, K, n, K.SURROGATE, args.drift_steps)
+    return out
+
+
+def hybrid_leg(args):
+    """Config-5 slab: 1,048,576 argon atoms (simple cubic 256 x 64 x 64, rho 0.8, 4:1:1 box);
+    exact ATM in the central 20 % of x, cos^2 blend over 2 sigma, surrogate elsewhere."""
+    import komd_amd as K
+    from komd_amd import systems as S
+    sysm = S.config_slab_1048576()
+    n = sysm.n
+    Lx = float(sysm.box[0])
+    center, half, blend = float(sysm.low[0]) + 0.5 * Lx, 0.1 * Lx, 2.0
+    out = {"n_atoms": n, "box": [float(x) for x in sysm.box], "exact_slab": [center - half, center + half],
+           "blend_width": blend}
+    with K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU) as ctx:
+        ctx.surrogate_set_params(np.load(WEIGHTS))
+        ctx.set_hybrid(0, center, half, blend)
+        ctx.compute_forces(K.HYBRID)
+        ctx.run(200, kind3b=K.HYBRID, t0=0.8)   # melt the lattice
+        r0, v0, _ = ctx.get_state()
+        # per-region three-body force error against the exact ATM everywhere
+        ctx.update_container()
+        ctx.forces_3b(K.ATM_EXACT)
+        f_atm = ctx.forces()
+        ctx.update_container()
+        ctx.forces_3b(K.HYBRID)
+        f_hyb = ctx.forces()
+        rw, _, _ = ctx.get_state()
+        d = rw[:, 0] - center
+        d -= Lx * np.rint(d / Lx)
+        ad = np.abs(d)
+        regions = {"exact": ad <= half, "blend": (ad > half) & (ad < half + blend), "surrogate": ad >= half + blend}
+        scale = float(np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1))))
+        err = {}
+        for name, m in regions.items():
+            e = np.sqrt(np.mean(np.sum((f_hyb[m] - f_atm[m]) ** 2, axis=1))) / scale if m.any() else None
+            err[name] = {"sites": int(m.sum()), "three_body_rms_error": e}
+        out["regions"] = err
+        legs = {}
+        for kind, name in ((K.ATM_EXACT, "exact_everywhere"), (K.HYBRID, "hybrid")):
+            ctx.set_state(r0, v0)
+            ctx.compute_forces(kind)
+            ctx.run(5, kind3b=kind)
+            t0 = time.perf_counter()
+            ctx.run(args.hybrid_steps, kind3b=kind)
+            dt = time.perf_counter() - t0
+            ctx.timing(True)
+            ctx.run(min(args.hybrid_steps, 20), kind3b=kind)
+            kern = {}
+            for k in ("cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
+                ms, cnt = ctx.timer(k)
+                if cnt:
+                    kern[k] = round(ms / cnt * 1e3, 2)
+            ctx.timing(False)
+            legs[name] = {"value": round(n * args.hybrid_steps / dt, 1),
+                          "ms_per_step": round(dt / args.hybrid_steps * 1e3, 4), "kernels_us": kern}
+        out.update(legs)
+        k = legs["hybrid"]["kernels_us"]
+        ns_exact = err["exact"]["sites"] + err["blend"]["sites"]
+        ns_sur = err["surrogate"]["sites"] + err["blend"]["sites"]
+        # region throughput: the sites a region's three-body kernels serve over those kernels' time
+        out["region_three_body_throughput"] = {
+            "exact_atm_kernel": {"sites": ns_exact, "us": k.get("atm"),
+                                 "site_steps_per_s": round(ns_exact / (k["atm"] * 1e-6), 1) if k.get("atm") else None},
+            "surrogate_kernels": {"sites": ns_sur,
+                                  "us": round(sum(k.get(x, 0.0) for x in ("sur_desc", "sur_mlp", "sur_force",
+                                                                          "sur_apply")), 2)}}
+        su = out["region_three_body_throughput"]["surrogate_kernels"]
+        su["site_steps_per_s"] = round(ns_sur / (su["us"] * 1e-6), 1) if su["us"] else None
+        out["speedup_hybrid_vs_exact"] = round(legs["hybrid"]["value"] / legs["exact_everywhere"]["value"], 3)
     return out
 
 
@@ -257,6 +329,10 @@ def main():
         for na in [int(x) for x in args.learned_atoms.split(",") if x]:
             learned[str(na)] = learned_leg(na, args)
 
+    hybrid = None
+    if not args.no_hybrid and os.path.exists(WEIGHTS):
+        hybrid = hybrid_leg(args)
+
     # ---- CPU baseline: FP64 OpenMP restatement of the reference loop, bounded sample
     cpu = None
     if not args.no_cpu:
@@ -288,6 +364,7 @@ def main():
         "energy_drift": drift,
         "kernels": kernels,
         "learned": learned or None,
+        "hybrid_slab": hybrid,
     }
     print(json.dumps(out), flush=True)
 

@@ -188,6 +188,8 @@ struct DescArgs {
     const float* __restrict__ params;  // shift / scale (nullptr: raw descriptors)
     float* __restrict__ X;             // [n][48]
     float* __restrict__ T;             // [n][76]
+    const float* __restrict__ lam;     // hybrid: sites with lambda = 1 carry no surrogate energy (skipped)
+    const uint32_t* __restrict__ sorted;
     int n;
     float pirc;
 };
@@ -201,7 +203,8 @@ __global__ __launch_bounds__(256) void k_sur_desc(DescArgs a) {
     for (int k = 0; k < NR; k++) R[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < NT; k++) M[k] = 0.f;
-    const int nn = (int)a.nlc[p];
+    int nn = (int)a.nlc[p];
+    if (a.lam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // weight 1 - lambda = 0: descriptors unused
     const float4* row = a.nl + (size_t)p * kFullCap;
     // neighbour entries are loaded one iteration ahead of their use
     float4 vnext = sub < nn ? row[sub] : make_float4(1.f, 0.f, 0.f, 0.f);
@@ -534,6 +537,7 @@ struct ForceArgs {
     const float* __restrict__ S;
     const float* __restrict__ E;      // hybrid: per-site surrogate energies (cell-sorted)
     const float* __restrict__ dlam;   // hybrid: d lambda / d axis coordinate (site order)
+    const float* __restrict__ lam;    // hybrid: lambda (site order)
     float4* __restrict__ P;           // partial rows [ncell][27][kRankCap]
     uint32_t* __restrict__ flags;
     double* __restrict__ fx;
@@ -605,7 +609,8 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
 #pragma unroll
         for (int k = 0; k < 80; k++) Ai[k] = si[NR + k];
         const int p = hs + h;
-        const int nn = (int)a.nlc[p];
+        int nn = (int)a.nlc[p];
+        if (a.dlam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // hybrid: no surrogate energy on this site
         const float4* row = a.nl + (size_t)p * kFullCap;
         const uint8_t* drow = a.nld + (size_t)p * kFullCap;
         float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -1215,10 +1220,10 @@ int site_blocks(komd_ctx* c) { return (int)((c->n + 255) / 256); }
 int group_blocks(komd_ctx* c) { return (int)((c->n * kLps + 255) / 256); }  // kLps lanes per site
 float pirc_of(komd_ctx* c) { return (float)(M_PI / c->prm.cutoff); }
 
-hipError_t launch_desc(komd_ctx* c, bool raw) {
+hipError_t launch_desc(komd_ctx* c, bool raw, const float* lam = nullptr) {
     ScopedTimer tm(c, "sur_desc");
     Surrogate* s = c->sur;
-    DescArgs a{c->d.nl, c->d.nlc, raw ? nullptr : s->params, s->X, s->T, (int)c->n, pirc_of(c)};
+    DescArgs a{c->d.nl, c->d.nlc, raw ? nullptr : s->params, s->X, s->T, lam, c->d.sorted, (int)c->n, pirc_of(c)};
     k_sur_desc<<<group_blocks(c), 256, 0, c->stream>>>(a);
     return hipGetLastError();
 }
@@ -1250,9 +1255,13 @@ hipError_t surrogate_pass(komd_ctx* c, bool energy, bool hybrid, int mode) {
     Surrogate* s = c->sur;
     if (!s || !s->have_params) return hipErrorNotSupported;
     KOMD_CHECK_HIP(ensure_site_buffers(c));
-    KOMD_CHECK_HIP(launch_desc(c, false));
+    KOMD_CHECK_HIP(launch_desc(c, false, hybrid ? s->lam : nullptr));
     KOMD_CHECK_HIP(launch_mlp(c, energy, hybrid ? s->lam : nullptr));
     ForceArgs a = force_args(c);
+    if (hybrid) {
+        a.dlam = s->dlam;
+        a.lam = s->lam;
+    }
     {
         ScopedTimer tm(c, "sur_force");
         k_sur_force<<<8 * a.chunk, 256, 0, c->stream>>>(a);
@@ -1263,7 +1272,6 @@ hipError_t surrogate_pass(komd_ctx* c, bool energy, bool hybrid, int mode) {
         a.fout = s->fj;
         k_sur_apply<FORCE_OUT><<<site_blocks(c), 256, 0, c->stream>>>(a);
     } else {
-        if (hybrid) a.dlam = s->dlam;
         k_sur_apply<FORCE_APPLY><<<site_blocks(c), 256, 0, c->stream>>>(a);
     }
     return hipGetLastError();
