@@ -524,7 +524,6 @@ __global__ __launch_bounds__(256) void k_sur_mlp(MlpArgs a) {
 // cells that see it) and hands the force to its consumer.  No state is gathered, no
 // global atomic is used, and the result does not depend on scheduling.
 enum { FORCE_APPLY = 0, FORCE_TRAIN = 1, FORCE_OUT = 2 };
-constexpr int kRankCap = 64;   // sites per cell held by a partial row
 
 struct ForceArgs {
     const float4* __restrict__ nl;
@@ -538,7 +537,7 @@ struct ForceArgs {
     const float* __restrict__ E;      // hybrid: per-site surrogate energies (cell-sorted)
     const float* __restrict__ dlam;   // hybrid: d lambda / d axis coordinate (site order)
     const float* __restrict__ lam;    // hybrid: lambda (site order)
-    float4* __restrict__ P;           // partial rows [ncell][27][kRankCap]
+    float4* __restrict__ P;           // partial rows [ncell][27][kRows]
     uint32_t* __restrict__ flags;
     double* __restrict__ fx;
     double* __restrict__ fy;
@@ -562,7 +561,7 @@ __device__ __forceinline__ int wrap_cell(const Grid& g, int x, int y, int z) {
 __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
     __shared__ int nb_start[27], nb_cnt[27], nb_base[28];
     __shared__ float acc[3][kMaxM];
-    __shared__ float hst[kRankCap * NS];
+    __shared__ float hst[kRows * NS];
     __shared__ int bad;
     const int tid = threadIdx.x, b = blockIdx.x;
     const int cell = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous slab of cells per XCD
@@ -579,7 +578,7 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
         nb_base[0] = 0;
         for (int k = 0; k < 27; k++) {
             nb_base[k + 1] = nb_base[k] + nb_cnt[k];
-            if (nb_cnt[k] > kRankCap) ok = 0;
+            if (nb_cnt[k] > kRows) ok = 0;
         }
         if (nb_base[27] > kMaxM) ok = 0;
         bad = !ok;
@@ -666,11 +665,11 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
         }
     }
     __syncthreads();
-    float4* out = a.P + (size_t)cell * 27 * kRankCap;
+    float4* out = a.P + (size_t)cell * 27 * kRows;
     for (int d = 0; d < 27; d++)
         for (int r = tid; r < nb_cnt[d]; r += 256) {
             const int e = nb_base[d] + r;
-            out[d * kRankCap + r] = make_float4(acc[0][e], acc[1][e], acc[2][e], 0.f);
+            out[d * kRows + r] = make_float4(acc[0][e], acc[1][e], acc[2][e], 0.f);
         }
 }
 
@@ -684,12 +683,12 @@ __global__ __launch_bounds__(256) void k_sur_apply(ForceArgs a) {
         const int r = p - (int)a.start[cb];
         const int bx = cb % a.g.nx, by = (cb / a.g.nx) % a.g.ny, bz = cb / (a.g.nx * a.g.ny);
         float4 F = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r < kRankCap) {  // larger cells are reported by the force pass
+        if (r < kRows) {  // larger cells are reported by the force pass
 #pragma unroll
             for (int d = 0; d < 27; d++) {
                 // the cell that sees this one in direction d lies at offset -d
                 const int ca = wrap_cell(a.g, bx - (d % 3 - 1), by - ((d / 3) % 3 - 1), bz - (d / 9 - 1));
-                const float4 t = a.P[((size_t)ca * 27 + d) * kRankCap + r];
+                const float4 t = a.P[((size_t)ca * 27 + d) * kRows + r];
                 F.x += t.x;
                 F.y += t.y;
                 F.z += t.z;
@@ -1156,8 +1155,6 @@ struct Surrogate {
     bool have_params = false;
     int64_t n = 0;
     float *X = nullptr, *T = nullptr, *S = nullptr, *E = nullptr;
-    float4* P = nullptr;   // partial force rows [ncell][27][kRankCap]
-    int P_cells = 0;
     // training
     float *mom = nullptr, *vel = nullptr, *gpart = nullptr, *grad = nullptr, *W = nullptr, *fj = nullptr;
     float4* delta = nullptr;
@@ -1201,12 +1198,6 @@ hipError_t ensure_site_buffers(komd_ctx* c) {
     KOMD_CHECK_HIP(salloc(&s->T, n * NT));
     KOMD_CHECK_HIP(salloc(&s->S, n * NS));
     KOMD_CHECK_HIP(salloc(&s->E, n));
-    if (s->P_cells != c->grid.ncell) {
-        if (s->P) hipFree(s->P);
-        s->P = nullptr;
-        KOMD_CHECK_HIP(salloc(&s->P, (size_t)c->grid.ncell * 27 * kRankCap));
-        s->P_cells = c->grid.ncell;
-    }
     KOMD_CHECK_HIP(salloc(&s->params, NPARAM));
     KOMD_CHECK_HIP(salloc(&s->ops, NOPS));
     return hipSuccess;
@@ -1240,7 +1231,7 @@ ForceArgs force_args(komd_ctx* c) {
     Surrogate* s = c->sur;
     ForceArgs a{};
     a.nl = c->d.nl; a.nlc = c->d.nlc; a.nld = c->d.nld; a.inv = c->d.inv; a.sorted = c->d.sorted;
-    a.key = c->d.key; a.start = c->d.start; a.S = s->S; a.E = s->E; a.P = s->P; a.flags = c->d.flags;
+    a.key = c->d.key; a.start = c->d.start; a.S = s->S; a.E = s->E; a.P = c->d.rows; a.flags = c->d.flags;
     a.g = c->grid;
     a.chunk = (c->grid.ncell + 7) / 8;
     a.fx = c->d.fx; a.fy = c->d.fy; a.fz = c->d.fz;
@@ -1331,7 +1322,7 @@ hipError_t surrogate_prepare(komd_ctx* c, bool hybrid) {
 void surrogate_free(komd_ctx* c) {
     Surrogate* s = c->sur;
     if (!s) return;
-    void* ptrs[] = {s->params, s->ops, s->X, s->T, s->S, s->E, s->P, s->mom, s->vel, s->gpart, s->grad, s->W, s->fj,
+    void* ptrs[] = {s->params, s->ops, s->X, s->T, s->S, s->E, s->mom, s->vel, s->gpart, s->grad, s->W, s->fj,
                     s->delta, s->lpart, s->lacc, s->save, s->stats, s->lam, s->dlam};
     for (void* p : ptrs)
         if (p) hipFree(p);

@@ -64,7 +64,7 @@ hipError_t dalloc(T** p, size_t count) {
 void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
-                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.red, d.cnt_pair, d.cnt_trip,
+                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
                     d.esite};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -154,6 +154,7 @@ hipError_t ensure_nl(komd_ctx* c, bool dirs) {
     if (!d.nl) KOMD_CHECK_HIP(dalloc(&d.nl, (size_t)c->n * kFullCap));
     if (!d.nlc) KOMD_CHECK_HIP(dalloc(&d.nlc, (size_t)c->n));
     if (dirs && !d.nld) KOMD_CHECK_HIP(dalloc(&d.nld, (size_t)c->n * kFullCap));
+    if (dirs && !d.rows) KOMD_CHECK_HIP(dalloc(&d.rows, (size_t)c->grid.ncell * 27 * kRows));
     return hipSuccess;
 }
 }  // namespace

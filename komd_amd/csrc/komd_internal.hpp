@@ -74,6 +74,7 @@ struct DevState {
     unsigned long long* cnt_pair = nullptr;  // [cell blocks] in-range pairs per workgroup slot
     unsigned long long* cnt_trip = nullptr;  // [ATM blocks] triplet visits per workgroup slot
     double* esite = nullptr;      // per-site energy scratch [N]
+    float4* rows = nullptr;       // per-cell partial force rows [C][27][kRows] (surrogate force pass)
 };
 
 struct Timer {

@@ -235,6 +235,6 @@ def test_graph_replays_match_eager(orc):
             b.timing(False)
             ra, va, _ = a.get_state()
             rb, vb, _ = b.get_state()
-            # FP32 LDS accumulation order is not fixed: differences stay at rounding level,
-            # a stale or missing graph node would show up as O(1e-3)
-            assert np.abs(ra - rb).max() < 1e-9 and np.abs(va - vb).max() < 2e-7
+            # FP32 LDS accumulation order (pair pass, three-body rows) is not fixed: differences
+            # stay at rounding level, a stale or missing graph node would show up as O(1e-3)
+            assert np.abs(ra - rb).max() < 1e-8 and np.abs(va - vb).max() < 1e-6
