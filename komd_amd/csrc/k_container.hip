@@ -134,6 +134,54 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(int ncell, uint32_t* __re
     if (t == kScanThreads - 1) start[ncell] = part[t];
 }
 
+// Large grids: tiles of 1024 cells, coalesced.  k_scan_tiles sums each tile;
+// k_scan_fin offsets a tile by the sum of the tiles before it, scans it in LDS,
+// writes start[] and zeroes the counts.
+constexpr int kScanTile = 1024;
+__global__ __launch_bounds__(kScanTile) void k_scan_tiles(int ncell, const uint32_t* __restrict__ count,
+                                                          uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t ws[kScanTile / 64];
+    const int i = blockIdx.x * kScanTile + threadIdx.x;
+    uint32_t v = i < ncell ? count[i] : 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int k = 0; k < kScanTile / 64; k++) s += ws[k];
+        tsum[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kScanTile) void k_scan_fin(int ncell, uint32_t* __restrict__ count,
+                                                        uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t part[kScanTile];
+    __shared__ uint32_t base;
+    const int t = threadIdx.x;
+    if (t < 64) {
+        uint32_t s = 0;
+        for (int k = t; k < (int)blockIdx.x; k += 64) s += tsum[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (t == 0) base = s;
+    }
+    const int i = blockIdx.x * kScanTile + t;
+    const uint32_t c = i < ncell ? count[i] : 0u;
+    part[t] = c;
+    __syncthreads();
+    for (int off = 1; off < kScanTile; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    if (i < ncell) {
+        start[i] = base + part[t] - c;
+        count[i] = 0;
+        if (i == ncell - 1) start[ncell] = base + part[t];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ slot,
                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
@@ -316,7 +364,15 @@ hipError_t launch_bin(komd_ctx* c) {
     const int n = (int)c->n;
     {
         ScopedTimer tm(c, "bin_scan");
-        k_scan<<<1, kScanThreads, 0, c->stream>>>(c->grid.ncell, d.count, d.start);
+        const int C = c->grid.ncell;
+        if (C <= 4 * kScanTile) {
+            k_scan<<<1, kScanThreads, 0, c->stream>>>(C, d.count, d.start);
+        } else {
+            const int tiles = (C + kScanTile - 1) / kScanTile;
+            k_scan_tiles<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.scanp);
+            KOMD_CHECK_HIP(hipGetLastError());
+            k_scan_fin<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.start, d.scanp);
+        }
         KOMD_CHECK_HIP(hipGetLastError());
     }
     {

@@ -64,7 +64,7 @@ hipError_t dalloc(T** p, size_t count) {
 void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
-                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
+                    d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.scanp, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
                     d.esite};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -354,6 +354,7 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     if (e == hipSuccess) e = dalloc(&d.mol_start, mstart.size());
     if (e == hipSuccess) e = dalloc(&d.count, C);
     if (e == hipSuccess) e = dalloc(&d.start, C + 1);
+    if (e == hipSuccess) e = dalloc(&d.scanp, (C + 1023) / 1024);
     if (e == hipSuccess) e = dalloc(&d.spos, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.sprm, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.red, red_n);

@@ -56,7 +56,7 @@ struct DevState {
     uint32_t* mol_start = nullptr;
     // binning
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *tmp = nullptr,
-             *sorted = nullptr, *inv = nullptr;
+             *sorted = nullptr, *inv = nullptr, *scanp = nullptr;
     float4* spos = nullptr;
     float2* sprm = nullptr;
     uint32_t* sid = nullptr;

@@ -35,8 +35,9 @@ def small(seed=11, cells=5, jitter=0.12, T=0.8):
     return S.fcc(cells, 0.8, T, seed, jitter=jitter)
 
 
-@pytest.mark.parametrize("cells,seed", [(5, 11), (10, 12), (20, 13)])
+@pytest.mark.parametrize("cells,seed", [(5, 11), (10, 12), (20, 13), (30, 14)])
 def test_binning_bit_exact(orc, cells, seed):
+    """cells 30: 20^3 = 8,000 domain cells, the tiled two-launch scan"""
     s = small(seed, cells)
     # push some sites outside the box so the wrap path runs
     s.r[::97] += np.array([s.box[0] * 0.999, -0.3, 0.0])
