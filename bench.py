@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--learned-atoms", default="32000,256000",
                     help="comma list of system sizes for the learned leg ('' to skip)")
     ap.add_argument("--no-hybrid", action="store_true", help="skip the 1,048,576-atom hybrid slab leg")
+    ap.add_argument("--no-other", action="store_true", help="skip the LJ-4000 and chains-262144 legs")
     ap.add_argument("--hybrid-steps", type=int, default=100)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
@@ -219,6 +220,65 @@ def hybrid_leg(args):
     return out
 
 
+def other_legs(args):
+    """The two-body configurations: plain LJ argon (4,000 atoms, 1,000 steps) and the
+    multi-site chains (65,536 x 4 sites, harmonic bonds and angles, LJ between molecules)."""
+    import komd_amd as K
+    from komd_amd import systems as S
+    out = {}
+    s1 = S.config_lj_4000()
+    with K.Context(s1, cutoff=2.5, dt=0.005) as ctx:
+        ctx.compute_forces(0)
+        o0 = ctx.observables(0)
+        t0 = time.perf_counter()
+        ctx.run(1000, kind3b=0)
+        dt = time.perf_counter() - t0
+        o1 = ctx.observables(0)
+        e0 = (o0["kinetic"] + o0["u_lj"]) / s1.n
+        e1 = (o1["kinetic"] + o1["u_lj"]) / s1.n
+        out["lj_4000"] = {"workload": "4,000 LJ argon atoms, FCC 10^3 x 4, rho 0.8, T* 1.0, rc 2.5, dt 0.005, NVE",
+                          "steps": 1000, "value": round(s1.n * 1000 / dt, 1), "ms_per_step": round(dt, 4),
+                          "energy_per_atom_start": e0, "energy_per_atom_end": e1}
+    s2 = S.config_chains_262144()
+    topo = {"bonds": s2.bonds, "angles": s2.angles}
+    # capped-force warm-up from the lattice (the clamp would also cap the stiff bonds later)
+    # sites binned on their own positions: a chain reaches ~1.2 sigma from its centre, so
+    # centre-of-mass cells of edge ~rc would miss pairs between neighbouring chains.
+    # Warm-up from the lattice with the force cap raised in stages (a single low cap lets
+    # pairs close to ~0.45 sigma, which the uncapped force then blows apart).
+    for lim in (20.0, 200.0, 2000.0, 2e4, 2e5):
+        with K.Context(s2, cutoff=2.5, dt=0.001, limit_factor=lim, topology=topo, bin_mode=K.BIN_SITE) as ctx:
+            ctx.compute_forces(0, bonded=True)
+            ctx.run(300, kind3b=0, bonded=True, t0=1.0)
+            s2.r, s2.v, _ = ctx.get_state()
+    with K.Context(s2, cutoff=2.5, dt=0.005, topology=topo, bin_mode=K.BIN_SITE) as ctx:
+        ctx.compute_forces(0, bonded=True)
+        ctx.run(args.warmup, kind3b=0, bonded=True, t0=1.0)
+        t0 = time.perf_counter()
+        ctx.run(args.steps, kind3b=0, bonded=True)
+        dt = time.perf_counter() - t0
+        ctx.timing(True)
+        ctx.run(100, kind3b=0, bonded=True)
+        kern = {}
+        for k in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj", "bonded", "kick"):
+            ms, cnt = ctx.timer(k)
+            if cnt:
+                kern[k] = round(ms / cnt * 1e3, 2)
+        ctx.timing(False)
+        obs = ctx.observables(0)
+        out["chains_262144"] = {
+            "workload": "65,536 linear 4-site chains (262,144 sites), site density 0.7, harmonic bonds k 500 r0 1, "
+                        "angles k 50 theta0 109.5 deg, LJ rc 2.5 between molecules, sites binned on their own "
+                        "positions, dt 0.005, NVE after a warm-up with the force cap raised from 20 to 2e5 "
+                        "in five 300-step stages at dt 0.001 with rescaling to T* 1.0",
+            "steps": args.steps, "value": round(s2.n * args.steps / dt, 1),
+            "ms_per_step": round(dt / args.steps * 1e3, 4), "kernels_us": kern,
+            "u_bond_per_site": obs["u_bond"] / s2.n, "u_angle_per_site": obs["u_angle"] / s2.n,
+            "temperature": obs["temperature"],
+            "equipartition": "3 bonds and 2 angles per 4 sites: kT/2 each gives 0.375 T and 0.25 T per site"}
+    return out
+
+
 def copy_calibration(mib: int):
     """A plain device copy of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE read on
     it give the factor that turns those counters into HBM bytes."""
@@ -332,6 +392,7 @@ def main():
     hybrid = None
     if not args.no_hybrid and os.path.exists(WEIGHTS):
         hybrid = hybrid_leg(args)
+    other = None if args.no_other else other_legs(args)
 
     # ---- CPU baseline: FP64 OpenMP restatement of the reference loop, bounded sample
     cpu = None
@@ -365,6 +426,7 @@ def main():
         "kernels": kernels,
         "learned": learned or None,
         "hybrid_slab": hybrid,
+        "two_body_configs": other,
     }
     print(json.dumps(out), flush=True)
 

@@ -189,8 +189,8 @@ def chains(nmol_xyz=(64, 32, 32), sites: int = 4, site_density: float = 0.7, bon
     gz, gy, gx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     centers = (np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1) + 0.5) * np.array(
         [length / nx, length / ny, length / nz])
-    r = centers[:, None, :] + np.einsum("mij,sj->msi", rot, tmpl)
-    r = np.mod(r.reshape(-1, 3), length)
+    # sites stay around their chain centre (the container wraps whole molecules by COM)
+    r = (centers[:, None, :] + np.einsum("mij,sj->msi", rot, tmpl)).reshape(-1, 3)
     mass = np.ones(n)
     v = maxwell_boltzmann(n, temperature, mass, seed + 7)
     mol = np.repeat(np.arange(nmol, dtype=np.uint64), sites)

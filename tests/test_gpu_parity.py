@@ -239,3 +239,25 @@ def test_graph_replays_match_eager(orc):
             # FP32 LDS accumulation order (pair pass, three-body rows) is not fixed: differences
             # stay at rounding level, a stale or missing graph node would show up as O(1e-3)
             assert np.abs(ra - rb).max() < 1e-8 and np.abs(va - vb).max() < 1e-6
+
+
+def test_site_binning_and_chain_lj(orc):
+    """KOMD_BIN_SITE: every site binned and wrapped on its own position (the oracle's
+    binning with one molecule per site), and LJ between different molecules equal to the
+    O(N^2) minimum-image oracle -- no pair is lost to centre-of-mass cells."""
+    s = S.chains((6, 6, 6), 4, 0.7, seed=12)
+    s.r = s.r + np.random.default_rng(3).normal(size=s.r.shape) * 0.02
+    single = s.copy()
+    single.mol_id = np.arange(s.n, dtype=np.uint64)
+    r_or, b = orc.update_container(single)
+    with K.Context(s, bin_mode=K.BIN_SITE) as c:
+        c.update_container()
+        site_cell, _, _, _ = c.get_cells()
+        r_gpu, _, _ = c.get_state()
+        c.forces_pair()
+        f = c.forces()
+    assert np.array_equal(r_gpu, r_or) and np.array_equal(site_cell, b.site_cell)
+    f_or, _ = orc.lj_brute(s, r_or)
+    mx, rms = rel_err(f, f_or)
+    print(f"chain LJ (site binning) max {mx:.3e} rms {rms:.3e}")
+    assert mx < 2e-4 and rms < 5e-6
