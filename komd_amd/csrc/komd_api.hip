@@ -73,6 +73,8 @@ void free_sites(komd_ctx* c) {
     keep.flags = d.flags;
     keep.counters = d.counters;
     keep.bonds = d.bonds;
+    keep.bond_mol = d.bond_mol;
+    keep.angle_mol = d.angle_mol;
     keep.angles = d.angles;
     d = keep;
     c->have_sites = false;
@@ -288,6 +290,8 @@ void komd_destroy(komd_ctx* c) {
     if (c->d.counters) hipFree(c->d.counters);
     if (c->d.bonds) hipFree(c->d.bonds);
     if (c->d.angles) hipFree(c->d.angles);
+    if (c->d.bond_mol) hipFree(c->d.bond_mol);
+    if (c->d.angle_mol) hipFree(c->d.angle_mol);
     for (auto& pe : c->pending) {
         hipEventDestroy(pe.second.first);
         hipEventDestroy(pe.second.second);
@@ -322,6 +326,7 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
         if (mol_id[i] != mol_id[i - 1]) mstart.push_back((uint32_t)i);
     mstart.push_back((uint32_t)n);
     c->nmol = (int64_t)mstart.size() - 1;
+    c->mol_start_h = mstart;
     c->multisite = c->nmol != n;
     std::map<uint64_t, uint32_t> cls;
     std::vector<uint32_t> idc((size_t)n);
@@ -438,13 +443,51 @@ komd_status komd_set_topology(komd_ctx* c, int64_t nb, const uint32_t* bonds, do
     for (int64_t i = 0; i < 3 * na; i++)
         if ((int64_t)angles[i] >= c->n) return fail(c, KOMD_ERR_ARG, "angle index out of range");
     drop_graph(c);
-    if (c->d.bonds) hipFree(c->d.bonds);
-    if (c->d.angles) hipFree(c->d.angles);
-    c->d.bonds = c->d.angles = nullptr;
+    void* old[] = {c->d.bonds, c->d.angles, c->d.bond_mol, c->d.angle_mol};
+    for (void* p : old)
+        if (p) hipFree(p);
+    c->d.bonds = c->d.angles = c->d.bond_mol = c->d.angle_mol = nullptr;
+    // Group the terms by molecule when every bond and angle lies inside one molecule: one
+    // thread per molecule then updates its own sites without atomics.
+    const std::vector<uint32_t>& ms = c->mol_start_h;
+    const int64_t nmol = c->nmol;
+    std::vector<uint32_t> mol_of((size_t)c->n);
+    for (int64_t m = 0; m < nmol; m++)
+        for (uint32_t i = ms[(size_t)m]; i < ms[(size_t)m + 1]; i++) mol_of[i] = (uint32_t)m;
+    bool by_mol = true;
+    for (int64_t t = 0; t < nb && by_mol; t++) by_mol = mol_of[bonds[2 * t]] == mol_of[bonds[2 * t + 1]];
+    for (int64_t t = 0; t < na && by_mol; t++)
+        by_mol = mol_of[angles[3 * t]] == mol_of[angles[3 * t + 1]] && mol_of[angles[3 * t + 1]] == mol_of[angles[3 * t + 2]];
+    std::vector<uint32_t> b_sorted(bonds, bonds + 2 * nb), a_sorted(angles, angles + 3 * na);
+    std::vector<uint32_t> b_start((size_t)nmol + 1, 0), a_start((size_t)nmol + 1, 0);
+    if (by_mol) {
+        // stable counting sort by molecule
+        for (int64_t t = 0; t < nb; t++) b_start[mol_of[bonds[2 * t]] + 1]++;
+        for (int64_t t = 0; t < na; t++) a_start[mol_of[angles[3 * t]] + 1]++;
+        for (int64_t m = 0; m < nmol; m++) {
+            b_start[(size_t)m + 1] += b_start[(size_t)m];
+            a_start[(size_t)m + 1] += a_start[(size_t)m];
+        }
+        std::vector<uint32_t> bp(b_start.begin(), b_start.end() - 1), ap(a_start.begin(), a_start.end() - 1);
+        for (int64_t t = 0; t < nb; t++) {
+            const uint32_t q = bp[mol_of[bonds[2 * t]]]++;
+            b_sorted[2 * q] = bonds[2 * t];
+            b_sorted[2 * q + 1] = bonds[2 * t + 1];
+        }
+        for (int64_t t = 0; t < na; t++) {
+            const uint32_t q = ap[mol_of[angles[3 * t]]]++;
+            for (int k = 0; k < 3; k++) a_sorted[3 * q + k] = angles[3 * t + k];
+        }
+        HIPCALL(c, dalloc(&c->d.bond_mol, b_start.size()), "komd_set_topology");
+        HIPCALL(c, dalloc(&c->d.angle_mol, a_start.size()), "komd_set_topology");
+        HIPCALL(c, hipMemcpy(c->d.bond_mol, b_start.data(), b_start.size() * 4, hipMemcpyHostToDevice), "komd_set_topology");
+        HIPCALL(c, hipMemcpy(c->d.angle_mol, a_start.data(), a_start.size() * 4, hipMemcpyHostToDevice), "komd_set_topology");
+    }
+    c->bonded_by_mol = by_mol;
     HIPCALL(c, dalloc(&c->d.bonds, (size_t)std::max<int64_t>(1, 2 * nb)), "komd_set_topology");
     HIPCALL(c, dalloc(&c->d.angles, (size_t)std::max<int64_t>(1, 3 * na)), "komd_set_topology");
-    if (nb) HIPCALL(c, hipMemcpy(c->d.bonds, bonds, (size_t)(2 * nb) * 4, hipMemcpyHostToDevice), "komd_set_topology");
-    if (na) HIPCALL(c, hipMemcpy(c->d.angles, angles, (size_t)(3 * na) * 4, hipMemcpyHostToDevice), "komd_set_topology");
+    if (nb) HIPCALL(c, hipMemcpy(c->d.bonds, b_sorted.data(), (size_t)(2 * nb) * 4, hipMemcpyHostToDevice), "komd_set_topology");
+    if (na) HIPCALL(c, hipMemcpy(c->d.angles, a_sorted.data(), (size_t)(3 * na) * 4, hipMemcpyHostToDevice), "komd_set_topology");
     c->nbonds = nb;
     c->nangles = na;
     c->kb = kb;

@@ -66,6 +66,7 @@ struct DevState {
     uint8_t* nld = nullptr;
     // bonded
     uint32_t *bonds = nullptr, *angles = nullptr;
+    uint32_t *bond_mol = nullptr, *angle_mol = nullptr;  // CSR of bonds / angles per molecule
     // reductions and flags
     double* red = nullptr;        // partials (see k_container.hip)
     double* scal = nullptr;       // device scalars: [0] mv2, [1] count, [2] beta, ...
@@ -93,6 +94,8 @@ struct komd_ctx {
     bool multisite = false;    // some molecule has more than one site
     float sig_u = 1.f, eps_u = 1.f;
     int64_t nbonds = 0, nangles = 0;
+    std::vector<uint32_t> mol_start_h;  // host copy of the molecule ranges
+    bool bonded_by_mol = false;         // every bond/angle inside one molecule: per-molecule kernel
     double kb = 0, r0 = 0, ka = 0, theta0 = 0;
     hipStream_t stream = nullptr;
     komd::DevState d;
