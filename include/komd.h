@@ -80,11 +80,22 @@ typedef struct {
     double u_3b;           /* three-body energy (exact ATM or surrogate)        */
     double u_bond;         /* harmonic bond energy                              */
     double u_angle;        /* harmonic angle energy                             */
-    double virial;         /* sum over pairs and triplets of r_i . F_i          */
+    double virial;         /* trace of virial_tensor                            */
     double fmax;           /* largest |F| component                             */
     int64_t n_in_box;      /* sites inside [low, high] (temperature count)      */
     int64_t pairs;         /* unique LJ pairs within cutoff at the last pass    */
     int64_t triplets;      /* unique ATM triplets within cutoff at the last pass */
+    /* Stress tensor parts, components XX XY YY XZ YZ ZZ (PressureSensor.h:18).
+     * kinetic_tensor: sum_i m v_a v_b (PressureSensor.cpp:142-150).
+     * virial_tensor: sum over interactions of sum_members r_a F_b, from relative
+     * vectors (periodic-safe): LJ pairs dr_a F_b (PressureSensor.cpp:152-174);
+     * ATM triplets with the physical sign for both ATM kinds, zz included
+     * (PressureSensor.cpp:223-287 omits zz); harmonic bonds and angles; the
+     * surrogate -sum_ij r_ij,a dE_i/dr_ij,b.  The hybrid blend's -V grad(w) and
+     * E grad(lambda) terms act as external fields and are not included.
+     * PressureSensor::write divides (kinetic + virial) by the volume. */
+    double kinetic_tensor[6];
+    double virial_tensor[6];
 } komd_observables;
 
 typedef struct komd_ctx komd_ctx;
@@ -156,6 +167,15 @@ komd_status komd_rescale(komd_ctx* ctx, double t0);
  * Recomputes the force passes in energy mode; forces are left as they were.
  * kind3b: 0 none, else a KOMD_3B_* kind.  Replaces LJ12_6_Sensor / PressureSensor. */
 komd_status komd_measure_observables(komd_ctx* ctx, int32_t kind3b, komd_observables* out);
+
+/* One sample of RDFSensor::measure (RDFSensor.cpp:23-35, kernel :59-72): for every
+ * ordered pair of distinct molecules, the minimum-image distance r of their
+ * centres of mass (delta - L round(delta / L) per axis, MoleculeContainer.cpp:
+ * 55-91 for the COM); when r <= max_r, one count in bin clamp((uint64)(r/delta_r),
+ * 0, nbins - 1).  nbins must equal (uint64)(max_r / delta_r) (RDFSensor.cpp:19),
+ * at most 8192.  counts receives this sample's nbins totals (overwritten);
+ * RDFSensor::write's averaging and g(r) normalisation stay with the caller. */
+komd_status komd_rdf(komd_ctx* ctx, double max_r, double delta_r, int64_t nbins, uint64_t* counts);
 
 /* Simulation::run main loop (Simulation.cpp:49-74) on the device: nsteps x
  * {integrate0, update_container, LJ, [bonded], [limit], [3-body], integrate1,

@@ -47,10 +47,15 @@ class Params(C.Structure):
 class Observables(C.Structure):
     _fields_ = [("kinetic", C.c_double), ("temperature", C.c_double), ("u_lj", C.c_double), ("u_3b", C.c_double),
                 ("u_bond", C.c_double), ("u_angle", C.c_double), ("virial", C.c_double), ("fmax", C.c_double),
-                ("n_in_box", C.c_int64), ("pairs", C.c_int64), ("triplets", C.c_int64)]
+                ("n_in_box", C.c_int64), ("pairs", C.c_int64), ("triplets", C.c_int64),
+                ("kinetic_tensor", C.c_double * 6), ("virial_tensor", C.c_double * 6)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = np.array(v[:], np.float64) if k.endswith("_tensor") else v
+        return out
 
 
 # Every entry point of include/komd.h with its ctypes signature.
@@ -77,6 +82,7 @@ _SIGNATURES = {
     "komd_measure_temperature": (C.c_int, [_P, _P]),
     "komd_rescale": (C.c_int, [_P, C.c_double]),
     "komd_measure_observables": (C.c_int, [_P, C.c_int32, _P]),
+    "komd_rdf": (C.c_int, [_P, C.c_double, C.c_double, C.c_int64, _P]),
     "komd_run": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int32, C.c_double]),
     "komd_compute_forces": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "komd_timing_enable": (C.c_int, [_P, C.c_int32]),
@@ -134,6 +140,26 @@ def device_name() -> str:
     if st != KOMD_OK:
         raise KomdError(st, "no HIP device")
     return buf.value.decode()
+
+
+def rdf_bins(max_r: float, delta_r: float) -> int:
+    """Bin count of RDFSensor (RDFSensor.cpp:19): the double quotient truncated to uint64."""
+    q = float(max_r) / float(delta_r)
+    if not (q >= 1.0):
+        raise ValueError("rdf: max_r / delta_r must be at least 1")
+    return int(q)
+
+
+def rdf_normalise(counts: np.ndarray, samples: int, n_mol: int, volume: float, delta_r: float):
+    """RDFSensor::write (RDFSensor.cpp:37-57): bin centres r and g(r) from accumulated
+    ordered-pair counts, g = (n_bin V) / (V_shell n (n - 1)), n_bin = counts / samples."""
+    nb = counts.shape[0]
+    h = delta_r / 2.0
+    r = np.add.accumulate(np.r_[h, np.full(nb - 1, float(delta_r))]) if nb > 1 else np.array([h])
+    v_bin = 4.0 / 3.0 * np.pi * ((r + h) ** 3 - (r - h) ** 3)
+    n_bin = counts.astype(np.float64) / samples
+    g = (n_bin * volume) / (v_bin * (float(n_mol) * (n_mol - 1)))
+    return r, g
 
 
 def _p(a: np.ndarray):
@@ -286,6 +312,14 @@ class Context:
         o = Observables()
         self._check(self.lib.komd_measure_observables(self._ctx, kind3b, C.byref(o)))
         return o.as_dict()
+
+    def rdf(self, max_r: float = 5.0, delta_r: float = 0.1) -> np.ndarray:
+        """One RDFSensor::measure sample (RDFSensor.cpp:23-35): ordered-pair counts per bin
+        of the molecules' minimum-image COM distances, (uint64)(max_r / delta_r) bins."""
+        nbins = rdf_bins(max_r, delta_r)
+        out = np.zeros(nbins, np.uint64)
+        self._check(self.lib.komd_rdf(self._ctx, float(max_r), float(delta_r), nbins, _p(out)))
+        return out
 
     # -- measurement
     def timing(self, enable: bool):

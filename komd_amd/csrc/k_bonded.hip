@@ -1,5 +1,6 @@
 // Harmonic bond and angle forces, force clamp -- gfx950.
 #include "komd_internal.hpp"
+#include "komd_reduce.hpp"
 
 namespace komd {
 namespace {
@@ -13,8 +14,10 @@ __global__ __launch_bounds__(256) void k_bonds(int nb, const uint32_t* __restric
                                                const double* __restrict__ rx, const double* __restrict__ ry,
                                                const double* __restrict__ rz, double* __restrict__ fx,
                                                double* __restrict__ fy, double* __restrict__ fz, Grid g,
-                                               double* __restrict__ part, int energy) {
+                                               double* __restrict__ part, double* __restrict__ vpart, int vparts,
+                                               int energy) {
     double u = 0;
+    float w[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
         const uint32_t i = bonds[2 * t], j = bonds[2 * t + 1];
         const float dx = min_image(rx[j] - rx[i], g.size[0]);
@@ -29,9 +32,13 @@ __global__ __launch_bounds__(256) void k_bonds(int nb, const uint32_t* __restric
         unsafeAtomicAdd(&fx[i], (double)(-gfac * dx));
         unsafeAtomicAdd(&fy[i], (double)(-gfac * dy));
         unsafeAtomicAdd(&fz[i], (double)(-gfac * dz));
-        if (energy) u += 0.5 * (double)(kb * dl * dl);
+        if (energy) {
+            u += 0.5 * (double)(kb * dl * dl);
+            sym_outer<float>(w, dx, dy, dz, gfac * dx, gfac * dy, gfac * dz);  // (r_j - r_i) (x) F_j
+        }
     }
     if (energy) {
+        block_store_sym(vpart, vparts, blockIdx.x, w);
         __shared__ double sh[256];
         sh[threadIdx.x] = u;
         __syncthreads();
@@ -47,8 +54,10 @@ __global__ __launch_bounds__(256) void k_angles(int na, const uint32_t* __restri
                                                 const double* __restrict__ rx, const double* __restrict__ ry,
                                                 const double* __restrict__ rz, double* __restrict__ fx,
                                                 double* __restrict__ fy, double* __restrict__ fz, Grid g,
-                                                double* __restrict__ part, int energy) {
+                                                double* __restrict__ part, double* __restrict__ vpart, int vparts,
+                                                int vslot0, int energy) {
     double u = 0;
+    float w[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na; t += gridDim.x * blockDim.x) {
         const uint32_t i = ang[3 * t], j = ang[3 * t + 1], k = ang[3 * t + 2];
         const float ax = min_image(rx[i] - rx[j], g.size[0]);
@@ -75,9 +84,14 @@ __global__ __launch_bounds__(256) void k_angles(int na, const uint32_t* __restri
         unsafeAtomicAdd(&fx[j], -(double)(fix + fkx));
         unsafeAtomicAdd(&fy[j], -(double)(fiy + fky));
         unsafeAtomicAdd(&fz[j], -(double)(fiz + fkz));
-        if (energy) u += 0.5 * (double)(ka * dth * dth);
+        if (energy) {
+            u += 0.5 * (double)(ka * dth * dth);
+            sym_outer<float>(w, ax, ay, az, fix, fiy, fiz);  // levers from the centre site j
+            sym_outer<float>(w, bx, by, bz, fkx, fky, fkz);
+        }
     }
     if (energy) {
+        block_store_sym(vpart, vparts, vslot0 + blockIdx.x, w);
         __shared__ double sh[256];
         sh[threadIdx.x] = u;
         __syncthreads();
@@ -100,8 +114,9 @@ __global__ __launch_bounds__(256) void k_bonded_mol(int nmol, const uint32_t* __
                                                     const double* __restrict__ ry, const double* __restrict__ rz,
                                                     double* __restrict__ fx, double* __restrict__ fy,
                                                     double* __restrict__ fz, Grid g, double* __restrict__ part,
-                                                    int energy) {
+                                                    double* __restrict__ vpart, int vparts, int energy) {
     double ub = 0, ua = 0;
+    float w[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < nmol; m += gridDim.x * blockDim.x) {
         for (uint32_t t = bstart[m]; t < bstart[m + 1]; t++) {
             const uint32_t i = bonds[2 * t], j = bonds[2 * t + 1];
@@ -117,7 +132,10 @@ __global__ __launch_bounds__(256) void k_bonded_mol(int nmol, const uint32_t* __
             fx[i] -= (double)(gfac * dx);
             fy[i] -= (double)(gfac * dy);
             fz[i] -= (double)(gfac * dz);
-            if (energy) ub += 0.5 * (double)(kb * dl * dl);
+            if (energy) {
+                ub += 0.5 * (double)(kb * dl * dl);
+                sym_outer<float>(w, dx, dy, dz, gfac * dx, gfac * dy, gfac * dz);
+            }
         }
         for (uint32_t t = astart[m]; t < astart[m + 1]; t++) {
             const uint32_t i = ang[3 * t], j = ang[3 * t + 1], k = ang[3 * t + 2];
@@ -145,10 +163,15 @@ __global__ __launch_bounds__(256) void k_bonded_mol(int nmol, const uint32_t* __
             fx[j] -= (double)(fix + fkx);
             fy[j] -= (double)(fiy + fky);
             fz[j] -= (double)(fiz + fkz);
-            if (energy) ua += 0.5 * (double)(ka * dth * dth);
+            if (energy) {
+                ua += 0.5 * (double)(ka * dth * dth);
+                sym_outer<float>(w, ax, ay, az, fix, fiy, fiz);
+                sym_outer<float>(w, bx, by, bz, fkx, fky, fkz);
+            }
         }
     }
     if (energy) {
+        block_store_sym(vpart, vparts, blockIdx.x, w);
         __shared__ double sh[2][256];
         sh[0][threadIdx.x] = ub;
         sh[1][threadIdx.x] = ua;
@@ -183,6 +206,8 @@ __global__ __launch_bounds__(256) void k_limit(int n, double lim, const double* 
 
 }  // namespace
 
+// Energy mode: U partials in d.red ([0, 256) bonds, [256, 512) angles) and the virial
+// tensor in d.vpart as kSym slots of 512 partials; the caller zeroes both first.
 hipError_t launch_bonded(komd_ctx* c, bool energy) {
     ScopedTimer tm(c, "bonded");
     auto& d = c->d;
@@ -191,17 +216,20 @@ hipError_t launch_bonded(komd_ctx* c, bool energy) {
         const int nmol = (int)c->nmol;
         k_bonded_mol<<<blocks, 256, 0, c->stream>>>(nmol, d.bond_mol, d.angle_mol, d.bonds, d.angles, (float)c->kb,
                                                      (float)c->r0, (float)c->ka, (float)c->theta0, d.rx, d.ry, d.rz,
-                                                     d.fx, d.fy, d.fz, c->grid, d.red, energy ? 1 : 0);
+                                                     d.fx, d.fy, d.fz, c->grid, d.red, d.vpart, 2 * blocks,
+                                                     energy ? 1 : 0);
         return hipGetLastError();
     }
     if (c->nbonds > 0) {
         k_bonds<<<blocks, 256, 0, c->stream>>>((int)c->nbonds, d.bonds, (float)c->kb, (float)c->r0, d.rx, d.ry, d.rz,
-                                               d.fx, d.fy, d.fz, c->grid, d.red, energy ? 1 : 0);
+                                               d.fx, d.fy, d.fz, c->grid, d.red, d.vpart, 2 * blocks,
+                                               energy ? 1 : 0);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     if (c->nangles > 0) {
         k_angles<<<blocks, 256, 0, c->stream>>>((int)c->nangles, d.angles, (float)c->ka, (float)c->theta0, d.rx, d.ry,
-                                                d.rz, d.fx, d.fy, d.fz, c->grid, d.red + blocks, energy ? 1 : 0);
+                                                d.rz, d.fx, d.fy, d.fz, c->grid, d.red + blocks, d.vpart, 2 * blocks,
+                                                blocks, energy ? 1 : 0);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     return hipSuccess;

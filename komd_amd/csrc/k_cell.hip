@@ -40,6 +40,7 @@
 // Hybrid: the triplet energy is weighted by w = (l_i + l_j + l_k)/3, so site i
 // also receives -(sum V/3) dl_i/dx_axis.
 #include "komd_cell.hpp"
+#include "komd_reduce.hpp"
 
 namespace komd {
 namespace {
@@ -61,6 +62,7 @@ struct CellArgs {
     uint32_t* __restrict__ nlc;    // [N] its length
     uint8_t* __restrict__ nld;     // [N][kFullCap] neighbour cell direction (reference-mode ATM)
     double* __restrict__ part;
+    double* __restrict__ vpart;    // [kSym][nblocks] virial tensor partials (energy mode)
     uint32_t* __restrict__ flags;
     unsigned long long* __restrict__ counters;
     Grid g;
@@ -123,6 +125,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     const int b = blockIdx.x;
     const int cell = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: each XCD gets a contiguous slab
     double u_acc = 0.0, w_acc = 0.0;
+    double vir[kSym] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // pair virial r_ij (x) F_ij (energy mode)
     unsigned long long npairs = 0;
     const bool live = cell < a.g.ncell;
     int m = 0, hs = 0, nh = 0;
@@ -248,8 +251,11 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                     ay -= (double)(dy * fac);
                     az -= (double)(dz * fac);
                     if (ENERGY) {
+                        // each pair is visited from both members: half of U and of the virial
                         u_acc += 0.5 * (double)(e4 * (s6 * s6 - s6));
                         w_acc += 0.5 * (double)(r2 * fac);
+                        const float hf = 0.5f * fac;
+                        sym_outer<double>(vir, dx, dy, dz, dx * hf, dy * hf, dz * hf);
                     }
                 }
             }
@@ -299,6 +305,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
             a.part[b] = s0;
             a.part[a.nblocks + b] = s1;
         }
+        block_store_sym(a.vpart, a.nblocks, b, vir);
     }
 }
 
@@ -318,6 +325,7 @@ struct AtmArgs {
     const float* __restrict__ lam;     // hybrid: exact-ATM weight per site
     const float* __restrict__ dlam;    // hybrid: d lambda / d axis coordinate per site
     double* __restrict__ part;
+    double* __restrict__ vpart;        // [kSym][nblocks] virial tensor partials (energy mode)
     unsigned long long* __restrict__ counters;
     Grid g;
     int n, chunk, nblocks, hyb_axis;
@@ -343,6 +351,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     const int lb = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous site slabs per XCD
     const int p = lb * kAtmWaves + wvu;
     double e_acc = 0.0;
+    float vir[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // triplet virial share of this lane's visits
     unsigned long long nvisit = 0;
     bool p_skip = false;
     if (p < a.n) {
@@ -392,10 +401,19 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 const float A = -(D + 5.f * Pp);
                 const float iX = __builtin_amdgcn_rcpf(X), iY = __builtin_amdgcn_rcpf(Y);
                 const float t1 = p2 * p3, t23 = p1 * (p3 + p2);
-                fx += K * (A * (xv.x * iX + yv.x * iY) - t1 * (xv.x + yv.x) - t23 * zx);
-                fy += K * (A * (xv.y * iX + yv.y * iY) - t1 * (xv.y + yv.y) - t23 * zy);
-                fz += K * (A * (xv.z * iX + yv.z * iY) - t1 * (xv.z + yv.z) - t23 * zz);
+                const float gx = K * (A * (xv.x * iX + yv.x * iY) - t1 * (xv.x + yv.x) - t23 * zx);
+                const float gy = K * (A * (xv.y * iX + yv.y * iY) - t1 * (xv.y + yv.y) - t23 * zy);
+                const float gz = K * (A * (xv.z * iX + yv.z * iY) - t1 * (xv.z + yv.z) - t23 * zz);
+                fx += gx;
+                fy += gy;
+                fz += gz;
                 ev += a.nu * D52 * (D + 3.f * Pp);  // unweighted V of this visit
+                if (ENERGY) {
+                    // (r_i - centroid) (x) F_i, r_i - centroid = -(x + y)/3; the three member
+                    // visits sum to sum_m r_m (x) F_m.  Physical sign in both modes.
+                    const float sc = ATM == 2 ? (1.f / 3.f) : (-1.f / 3.f);
+                    sym_outer<float>(vir, sc * (xv.x + yv.x), sc * (xv.y + yv.y), sc * (xv.z + yv.z), gx, gy, gz);
+                }
             };
             const unsigned long long below = (1ull << lane) - 1ull;
             int head = 0, tail = 0;
@@ -510,6 +528,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             a.part[b] = s;
         }
     }
+    if (ENERGY) block_store_sym(a.vpart, a.nblocks, b, vir);
 }
 
 template <bool U, bool M, bool LJ, int NL, bool E, bool LIM>
@@ -552,7 +571,7 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
     a.rx = d.rx; a.ry = d.ry; a.rz = d.rz; a.eps = d.eps; a.sig = d.sig;
     a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
     a.nl = d.nl; a.nlc = d.nlc; a.nld = d.nld;
-    a.part = d.red; a.flags = d.flags; a.counters = d.cnt_pair;
+    a.part = d.red; a.vpart = d.vpart; a.flags = d.flags; a.counters = d.cnt_pair;
     a.g = c->grid;
     a.chunk = (c->grid.ncell + 7) / 8;
     a.nblocks = 8 * a.chunk;
@@ -578,7 +597,7 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     AtmArgs a{};
     a.nl = d.nl; a.nlc = d.nlc; a.nld = d.nld; a.sorted = d.sorted;
     a.rx = d.rx; a.ry = d.ry; a.rz = d.rz; a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
-    a.esite = esite; a.lam = lam; a.dlam = dlam; a.part = d.red; a.counters = d.cnt_trip;
+    a.esite = esite; a.lam = lam; a.dlam = dlam; a.part = d.red; a.vpart = d.vpart; a.counters = d.cnt_trip;
     a.g = c->grid;
     a.n = (int)c->n;
     a.nblocks = atm_blocks(c);

@@ -7,6 +7,7 @@
 #pragma clang fp contract(off)
 
 #include "komd_internal.hpp"
+#include "komd_reduce.hpp"
 
 namespace komd {
 namespace {
@@ -287,6 +288,25 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(int n, double* __restrict
     rz[i] = rz[i] + z * dt;
 }
 
+// PressureSensor::MomentumKernel (PressureSensor.cpp:142-150): sum_i m v_a v_b over
+// all sites, components XX XY YY XZ YZ ZZ, per-block partials.
+__global__ __launch_bounds__(kBlock) void k_kin_tensor(int n, const double* __restrict__ vx,
+                                                       const double* __restrict__ vy,
+                                                       const double* __restrict__ vz,
+                                                       const double* __restrict__ mass, double* __restrict__ part) {
+    double w[kSym] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double x = vx[i], y = vy[i], z = vz[i], m = mass[i];
+        w[0] += x * x * m;
+        w[1] += x * y * m;
+        w[2] += y * y * m;
+        w[3] += x * z * m;
+        w[4] += y * z * m;
+        w[5] += z * z * m;
+    }
+    block_store_sym(part, gridDim.x, blockIdx.x, w);
+}
+
 // Deterministic sum of nparts partials (fixed order, one wave per slot).
 __device__ double sum_partials(const double* __restrict__ p, int nparts) {
     __shared__ double sh[kBlock];
@@ -436,6 +456,11 @@ hipError_t launch_beta_only(komd_ctx* c) {
     auto& d = c->d;
     k_rescale<<<1, kBlock, 0, c->stream>>>((int)c->n, d.vx, d.vy, d.vz, d.red, c->red_blocks, c->prm.temp_factor,
                                            c->run_t0, d.scal, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_kin_tensor(komd_ctx* c) {
+    k_kin_tensor<<<kKinBlocks, kBlock, 0, c->stream>>>((int)c->n, c->d.vx, c->d.vy, c->d.vz, c->d.mass, c->d.vpart);
     return hipGetLastError();
 }
 

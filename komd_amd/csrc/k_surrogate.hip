@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "komd_cell.hpp"
+#include "komd_reduce.hpp"
 
 namespace komd {
 
@@ -546,8 +547,9 @@ struct ForceArgs {
     float4* __restrict__ delta;       // training: dL/dF per site (cell-sorted)
     float* __restrict__ fout;         // FORCE_OUT: forces [n][3], site order
     double* __restrict__ part;        // training: per-block loss partials
+    double* __restrict__ vpart;       // energy: [kSym][vparts] virial partials
     Grid g;
-    int n, hyb_axis, chunk;
+    int n, hyb_axis, chunk, vparts;
     float pirc, cF, lF;
 };
 
@@ -558,6 +560,9 @@ __device__ __forceinline__ int wrap_cell(const Grid& g, int x, int y, int z) {
     return x + g.nx * (y + g.ny * z);
 }
 
+// VIR: also the virial sum over directed pairs of -(r_j - r_i) (x) c (c is the force on i
+// from E_i through the pair; -c acts on j), per-block partials in a.vpart.
+template <bool VIR>
 __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
     __shared__ int nb_start[27], nb_cnt[27], nb_base[28];
     __shared__ float acc[3][kMaxM];
@@ -565,7 +570,10 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
     __shared__ int bad;
     const int tid = threadIdx.x, b = blockIdx.x;
     const int cell = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous slab of cells per XCD
-    if (cell >= a.g.ncell) return;
+    if (cell >= a.g.ncell) {
+        if (VIR && tid < kSym) a.vpart[(size_t)tid * a.vparts + b] = 0.0;
+        return;
+    }
     const int hx = cell % a.g.nx, hy = (cell / a.g.nx) % a.g.ny, hz = cell / (a.g.nx * a.g.ny);
     if (tid < 27) {
         const int nc = wrap_cell(a.g, hx + tid % 3 - 1, hy + (tid / 3) % 3 - 1, hz + tid / 9 - 1);
@@ -585,8 +593,12 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
         if (!ok) atomicOr(a.flags, FLAG_CELL_OVERFLOW);
     }
     __syncthreads();
-    if (bad) return;
+    if (bad) {
+        if (VIR && tid < kSym) a.vpart[(size_t)tid * a.vparts + b] = 0.0;
+        return;
+    }
     const int m = nb_base[27], hs = nb_start[13], nh = nb_cnt[13], self = nb_base[13];
+    float wv[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int e = tid; e < m; e += 256) acc[0][e] = acc[1][e] = acc[2][e] = 0.f;
     {
         const float4* src = reinterpret_cast<const float4*>(a.S + (size_t)hs * NS);
@@ -646,6 +658,7 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
             fx += cx;
             fy += cy;
             fz += cz;
+            if (VIR) sym_outer<float>(wv, -v.x, -v.y, -v.z, cx, cy, cz);
             const int rk = pjn - nb_start[d];
             if ((unsigned)rk >= (unsigned)nb_cnt[d]) continue;  // lists and cells come from one binning
             const int slot = nb_base[d] + rk;
@@ -671,6 +684,13 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
             const int e = nb_base[d] + r;
             out[d * kRows + r] = make_float4(acc[0][e], acc[1][e], acc[2][e], 0.f);
         }
+    if (VIR) block_store_sym(a.vpart, a.vparts, b, wv);
+}
+
+// dst[k * stride] = src ? src[k] : 0 for k < kSym (the extra virial slot of a pass).
+__global__ void k_set_sym_slot(double* __restrict__ dst, int stride, const double* __restrict__ src) {
+    const int k = threadIdx.x;
+    if (k < kSym) dst[(size_t)k * stride] = src ? src[k] : 0.0;
 }
 
 template <int MODE>
@@ -1255,7 +1275,13 @@ hipError_t surrogate_pass(komd_ctx* c, bool energy, bool hybrid, int mode) {
     }
     {
         ScopedTimer tm(c, "sur_force");
-        k_sur_force<<<8 * a.chunk, 256, 0, c->stream>>>(a);
+        if (energy) {
+            a.vpart = c->d.vpart;
+            a.vparts = surrogate_virial_blocks(c);
+            k_sur_force<true><<<8 * a.chunk, 256, 0, c->stream>>>(a);
+        } else {
+            k_sur_force<false><<<8 * a.chunk, 256, 0, c->stream>>>(a);
+        }
         KOMD_CHECK_HIP(hipGetLastError());
     }
     ScopedTimer tm(c, "sur_apply");
@@ -1286,7 +1312,12 @@ hipError_t hybrid_weights(komd_ctx* c) {
 hipError_t launch_surrogate_forces(komd_ctx* c, bool energy, int hybrid) {
     (void)hybrid;
     KOMD_CHECK_HIP(surrogate_pass(c, energy, false, FORCE_APPLY));
-    if (energy) KOMD_CHECK_HIP(hipMemsetAsync(c->d.red + mlp_blocks(c), 0, sizeof(double), c->stream));
+    if (energy) {
+        KOMD_CHECK_HIP(hipMemsetAsync(c->d.red + mlp_blocks(c), 0, sizeof(double), c->stream));
+        const int vb = surrogate_virial_blocks(c);
+        k_set_sym_slot<<<1, 64, 0, c->stream>>>(c->d.vpart + (vb - 1), vb, nullptr);
+        KOMD_CHECK_HIP(hipGetLastError());
+    }
     return hipSuccess;
 }
 
@@ -1298,15 +1329,23 @@ hipError_t hybrid_forces(komd_ctx* c, bool energy) {
     if (!s || !s->have_params || c->hyb_blend <= 0) return hipErrorNotSupported;
     KOMD_CHECK_HIP(hybrid_weights(c));
     KOMD_CHECK_HIP(launch_atm(c, 1, energy, s->lam, s->dlam, nullptr));
-    if (energy) KOMD_CHECK_HIP(launch_reduce_parts(c, c->d.red, atm_blocks(c), 1, c->d.scal + 12));
+    if (energy) {
+        KOMD_CHECK_HIP(launch_reduce_parts(c, c->d.red, atm_blocks(c), 1, c->d.scal + 12));
+        KOMD_CHECK_HIP(launch_reduce_parts(c, c->d.vpart, atm_blocks(c), kSym, c->d.scal + 16));
+    }
     KOMD_CHECK_HIP(surrogate_pass(c, energy, true, FORCE_APPLY));
-    if (energy)
+    if (energy) {
         KOMD_CHECK_HIP(hipMemcpyAsync(c->d.red + mlp_blocks(c), c->d.scal + 12, sizeof(double),
                                       hipMemcpyDeviceToDevice, c->stream));
+        const int vb = surrogate_virial_blocks(c);
+        k_set_sym_slot<<<1, 64, 0, c->stream>>>(c->d.vpart + (vb - 1), vb, c->d.scal + 16);
+        KOMD_CHECK_HIP(hipGetLastError());
+    }
     return hipSuccess;
 }
 
 int surrogate_energy_blocks(komd_ctx* c) { return mlp_blocks(c) + 1; }
+int surrogate_virial_blocks(komd_ctx* c) { return cell_blocks(c) + 1; }
 
 hipError_t surrogate_prepare(komd_ctx* c, bool hybrid) {
     Surrogate* s = c->sur;
@@ -1407,7 +1446,7 @@ hipError_t train_step(komd_ctx* c, int64_t k, float wE, float wF, float lr, bool
         a.part = s->lpart + kTrainBlocks;
         a.cF = 2.f * wF / (3.f * (float)n);
         a.lF = wF / (3.f * (float)n);
-        k_sur_force<<<8 * a.chunk, 256, 0, c->stream>>>(a);
+        k_sur_force<false><<<8 * a.chunk, 256, 0, c->stream>>>(a);
         KOMD_CHECK_HIP(hipGetLastError());
         k_sur_apply<FORCE_TRAIN><<<fb, 256, 0, c->stream>>>(a);
         KOMD_CHECK_HIP(hipGetLastError());

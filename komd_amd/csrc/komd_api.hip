@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "komd_cell.hpp"
+#include "komd_reduce.hpp"
 
 namespace komd {
 hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev);
@@ -65,7 +66,7 @@ void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
                     d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.scanp, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
-                    d.esite};
+                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf};
     for (void* p : ptrs)
         if (p) hipFree(p);
     DevState keep;
@@ -268,12 +269,12 @@ komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx
     g.ncell = g.nx * g.ny * g.nz;
     if (c->prm.temp_factor <= 0) c->prm.temp_factor = 1.0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = dalloc(&c->d.scal, 16);
+    if (e == hipSuccess) e = dalloc(&c->d.scal, 32);
     if (e == hipSuccess) e = dalloc(&c->d.flags, 1);
     if (e == hipSuccess) e = dalloc(&c->d.counters, 2);
     if (e == hipSuccess) e = hipMemset(c->d.flags, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->d.counters, 0, 2 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->d.scal, 0, 16 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->d.scal, 0, 32 * sizeof(double));
     *out = c;
     if (e != hipSuccess) return hip_fail(c, e, "komd_create");
     return KOMD_OK;
@@ -363,6 +364,9 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     if (e == hipSuccess) e = dalloc(&d.spos, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.sprm, (size_t)n);
     if (e == hipSuccess) e = dalloc(&d.red, red_n);
+    if (e == hipSuccess)
+        e = dalloc(&d.vpart, (size_t)kSym * std::max<size_t>({(size_t)cell_blocks(c) + 1, (size_t)atm_blocks(c),
+                                                               (size_t)512, (size_t)kKinBlocks}));
     if (e == hipSuccess) e = dalloc(&d.cnt_pair, (size_t)cell_blocks(c));
     if (e == hipSuccess) e = dalloc(&d.cnt_trip, (size_t)atm_blocks(c));
     if (e == hipSuccess) e = hipMemset(d.cnt_pair, 0, (size_t)cell_blocks(c) * 8);
@@ -755,52 +759,64 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
     if (s) return s;
     HIPCALL(c, launch_com_wrap_key(c, false), "komd_measure_observables");
     HIPCALL(c, launch_bin(c), "komd_measure_observables");
-    double* out_dev = d.scal + 8;
+    double* out_dev = d.scal + 24;  // [24, 32): results of one reduction
     double vals[8];
+    // reduce `nslots` slots of `nparts` partials and copy them to vals
+    auto reduce = [&](const double* part, int nparts, int nslots) -> komd_status {
+        HIPCALL(c, launch_reduce_parts(c, part, nparts, nslots, out_dev), "komd_measure_observables");
+        komd_status st = sync_check(c, "komd_measure_observables");
+        if (st) return st;
+        HIPCALL(c, hipMemcpy(vals, out_dev, (size_t)nslots * sizeof(double), hipMemcpyDeviceToHost),
+                "komd_measure_observables");
+        return KOMD_OK;
+    };
+    // virial tensor partials of the pass just run, added to out->virial_tensor
+    auto add_virial = [&](int nparts) -> komd_status {
+        komd_status st = reduce(d.vpart, nparts, kSym);
+        if (st) return st;
+        for (int k = 0; k < kSym; k++) out->virial_tensor[k] += vals[k];
+        return KOMD_OK;
+    };
     const int atm = atm_code(kind3b);
     const int nl = nl_mode(kind3b);
     if (nl) HIPCALL(c, ensure_nl(c, nl == 2), "komd_measure_observables");
     HIPCALL(c, launch_cell(c, true, nl, true, false), "komd_measure_observables");
-    HIPCALL(c, launch_reduce_parts(c, d.red, cell_blocks(c), 2, out_dev), "komd_measure_observables");
-    s = sync_check(c, "komd_measure_observables");
-    if (s) return s;
-    HIPCALL(c, hipMemcpy(vals, out_dev, 2 * sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+    if ((s = reduce(d.red, cell_blocks(c), 1))) return s;
     out->u_lj = vals[0];
-    out->virial = vals[1];
+    if ((s = add_virial(cell_blocks(c)))) return s;
     if (atm) {
         HIPCALL(c, launch_atm(c, atm, true, nullptr, nullptr, nullptr), "komd_measure_observables");
-        HIPCALL(c, launch_reduce_parts(c, d.red, atm_blocks(c), 1, out_dev), "komd_measure_observables");
-        s = sync_check(c, "komd_measure_observables");
-        if (s) return s;
-        HIPCALL(c, hipMemcpy(vals, out_dev, sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+        if ((s = reduce(d.red, atm_blocks(c), 1))) return s;
         out->u_3b = vals[0];
+        if ((s = add_virial(atm_blocks(c)))) return s;
     }
     if (c->nbonds + c->nangles > 0) {
         HIPCALL(c, hipMemset(d.red, 0, 512 * sizeof(double)), "komd_measure_observables");
+        HIPCALL(c, hipMemset(d.vpart, 0, (size_t)kSym * 512 * sizeof(double)), "komd_measure_observables");
         HIPCALL(c, launch_bonded(c, true), "komd_measure_observables");
-        HIPCALL(c, launch_reduce_parts(c, d.red, 256, 2, out_dev), "komd_measure_observables");
-        s = sync_check(c, "komd_measure_observables");
-        if (s) return s;
-        HIPCALL(c, hipMemcpy(vals, out_dev, 2 * sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+        if ((s = reduce(d.red, 256, 2))) return s;
         out->u_bond = vals[0];
         out->u_angle = vals[1];
+        if ((s = add_virial(512))) return s;
     }
     if (kind3b && !atm) {
         hipError_t e = forces_3b(c, kind3b, true);
         if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
         HIPCALL(c, e, "komd_measure_observables");
-        HIPCALL(c, launch_reduce_parts(c, d.red, surrogate_energy_blocks(c), 1, out_dev), "komd_measure_observables");
-        s = sync_check(c, "komd_measure_observables");
-        if (s) return s;
-        HIPCALL(c, hipMemcpy(vals, out_dev, sizeof(double), hipMemcpyDeviceToHost), "komd_measure_observables");
+        if ((s = reduce(d.red, surrogate_energy_blocks(c), 1))) return s;
         out->u_3b = vals[0];
+        if ((s = add_virial(surrogate_virial_blocks(c)))) return s;
     }
+    out->virial = out->virial_tensor[0] + out->virial_tensor[2] + out->virial_tensor[5];
     int64_t cnt1[2];
     s = read_counters(c, cnt1, false);
     if (s) return s;
     out->pairs = cnt1[0] - cnt0[0];
     out->triplets = (cnt1[1] - cnt0[1]) / 3;  // every triplet is visited by its 3 members
-    // kinetic energy and temperature
+    // kinetic tensor, kinetic energy and temperature
+    HIPCALL(c, launch_kin_tensor(c), "komd_measure_observables");
+    if ((s = reduce(d.vpart, kKinBlocks, kSym))) return s;
+    for (int k = 0; k < kSym; k++) out->kinetic_tensor[k] = vals[k];
     HIPCALL(c, launch_temperature(c), "komd_measure_observables");
     s = sync_check(c, "komd_measure_observables");
     if (s) return s;
@@ -814,6 +830,22 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
     for (double x : fsave) fm = std::max(fm, fabs(x));
     out->fmax = fm;
     return komd_set_state(c, nullptr, nullptr, fsave.data());
+}
+
+komd_status komd_rdf(komd_ctx* c, double max_r, double delta_r, int64_t nbins, uint64_t* counts) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (!counts) return fail(c, KOMD_ERR_ARG, "rdf: counts is NULL");
+    if (!(max_r > 0) || !(delta_r > 0)) return fail(c, KOMD_ERR_ARG, "rdf: max_r and delta_r must be positive");
+    const double q = max_r / delta_r;
+    if (!(q >= 1.0) || !(q < (double)kRdfMaxBins + 1.0) || (int64_t)(uint64_t)q != nbins)
+        return fail(c, KOMD_ERR_ARG,
+                    "rdf: nbins must equal (uint64)(max_r / delta_r), between 1 and " + std::to_string(kRdfMaxBins));
+    HIPCALL(c, launch_rdf(c, max_r, delta_r, (int)nbins), "komd_rdf");
+    s = sync_check(c, "komd_rdf");
+    if (s) return s;
+    HIPCALL(c, hipMemcpy(counts, c->d.rdf, (size_t)nbins * sizeof(uint64_t), hipMemcpyDeviceToHost), "komd_rdf");
+    return KOMD_OK;
 }
 
 komd_status komd_timing_enable(komd_ctx* c, int32_t enable) {

@@ -7,6 +7,6 @@ constexpr int kCellThreads = 256;  // one workgroup (4 waves) per home cell
 constexpr int kMaxM = 768;         // staged neighbourhood capacity when the three-body pass runs
 constexpr int kChunkLJ = 768;      // staging chunk of the pair-only pass (any neighbourhood size)
 constexpr int kMaxOwn = 64;        // home sites per cell when the three-body pass runs
-constexpr int kFullCap = 96;
-constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row       // full neighbour list capacity per home site (three-body pass)
+constexpr int kFullCap = 96;       // full neighbour list capacity per home site (three-body pass)
+constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row
 }  // namespace komd

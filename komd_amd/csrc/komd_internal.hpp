@@ -69,14 +69,20 @@ struct DevState {
     uint32_t *bond_mol = nullptr, *angle_mol = nullptr;  // CSR of bonds / angles per molecule
     // reductions and flags
     double* red = nullptr;        // partials (see k_container.hip)
-    double* scal = nullptr;       // device scalars: [0] mv2, [1] count, [2] beta, ...
+    double* vpart = nullptr;      // [6][blocks] virial / kinetic tensor partials (energy passes)
+    double* scal = nullptr;       // device scalars [32]: [0] mv2, [1] count, [2] beta, ...
     uint32_t* flags = nullptr;
     unsigned long long* counters = nullptr;  // unused legacy slot
     unsigned long long* cnt_pair = nullptr;  // [cell blocks] in-range pairs per workgroup slot
     unsigned long long* cnt_trip = nullptr;  // [ATM blocks] triplet visits per workgroup slot
     double* esite = nullptr;      // per-site energy scratch [N]
     float4* rows = nullptr;       // per-cell partial force rows [C][27][kRows] (surrogate force pass)
+    // radial distribution sensor (allocated on first use)
+    double *comx = nullptr, *comy = nullptr, *comz = nullptr;  // [M] molecule centres of mass
+    unsigned long long* rdf = nullptr;                         // [kRdfMaxBins] ordered-pair counts
 };
+
+constexpr int kRdfMaxBins = 8192;  // u32 LDS histogram of 32 KiB per workgroup
 
 struct Timer {
     double total_ms = 0;
@@ -153,6 +159,10 @@ hipError_t launch_temperature(komd_ctx* c);     // kinetic partials -> scal[0..1
 hipError_t launch_rescale(komd_ctx* c, double t0);    // beta from scal, v *= beta
 hipError_t launch_beta_only(komd_ctx* c);             // beta = sqrt(run_t0/T) into scal[2]
 hipError_t launch_reduce_parts(komd_ctx* c, const double* part, int nparts, int nslots, double* out_dev);
+hipError_t launch_kin_tensor(komd_ctx* c);      // sum m v_a v_b partials -> d.vpart ([6][kKinBlocks])
+constexpr int kKinBlocks = 256;
+// k_sensors.hip: one RDF sample of the molecules' COM distances into d.rdf (nbins counts)
+hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins);
 // k_cell.hip.  launch_cell: LJ (+ clamp) on cell tiles; nl: 0 no neighbour list, 1 write
 // the full lists, 2 lists + cell directions.  launch_atm: per-site ATM from the lists;
 // atm: 1 physical, 2 reference; lam/dlam: hybrid weights (nullptr otherwise); esite:
@@ -169,6 +179,9 @@ hipError_t launch_limit(komd_ctx* c);
 hipError_t launch_surrogate_forces(komd_ctx* c, bool energy, int hybrid);
 hipError_t hybrid_forces(komd_ctx* c, bool energy);
 int surrogate_energy_blocks(komd_ctx* c);
+// With energy, the surrogate pass leaves its virial partials in d.vpart as kSym slots of
+// surrogate_virial_blocks partials (the last one holds the hybrid's exact-ATM share).
+int surrogate_virial_blocks(komd_ctx* c);
 hipError_t surrogate_prepare(komd_ctx* c, bool hybrid);  // allocates the pass buffers (not capturable)
 void surrogate_free(komd_ctx* c);
 

@@ -1,0 +1,47 @@
+// Deterministic block reductions of the energy / virial passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace komd {
+
+// Components of a symmetric 3x3 tensor in the order XX XY YY XZ YZ ZZ
+// (PressureSensor.h:18).
+constexpr int kSym = 6;
+
+// w += sym(a (x) b) = (a_p b_q + a_q b_p) / 2 per component.
+template <typename T>
+__device__ __forceinline__ void sym_outer(T* w, T ax, T ay, T az, T bx, T by, T bz) {
+    w[0] += ax * bx;
+    w[1] += T(0.5) * (ax * by + ay * bx);
+    w[2] += ay * by;
+    w[3] += T(0.5) * (ax * bz + az * bx);
+    w[4] += T(0.5) * (ay * bz + az * by);
+    w[5] += az * bz;
+}
+
+// Sum of v over the workgroup in a fixed order (blockDim.x a multiple of 64, at most
+// 1024); every thread returns the total.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[16];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int k = 0; k < nw; k++) s += sh[k];
+    return s;
+}
+
+// part[k * nparts + slot] = block total of w[k], k < kSym.
+template <typename T>
+__device__ __forceinline__ void block_store_sym(double* __restrict__ part, int nparts, int slot, const T* w) {
+#pragma unroll
+    for (int k = 0; k < kSym; k++) {
+        const double s = block_sum((double)w[k]);
+        if (threadIdx.x == 0) part[(size_t)k * nparts + slot] = s;
+    }
+}
+
+}  // namespace komd

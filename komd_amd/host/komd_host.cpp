@@ -72,6 +72,11 @@ bool readFile(const std::string& path, Configuration& config, std::vector<Compon
         else if (var == "hybrid_region") file >> config.hybrid_axis >> config.hybrid_center >> config.hybrid_half >> config.hybrid_blend;
         else if (var == "thermostat") file >> config.thermostat;
         else if (var == "checkpoint_precision") file >> config.checkpoint_precision;
+        else if (var == "enable_sensor_rdf") file >> config.enable_sensor_rdf;
+        else if (var == "enable_sensor_pres") file >> config.enable_sensor_pres;
+        else if (var == "sensor_rdf_max") file >> config.sensor_rdf_max;
+        else if (var == "sensor_rdf_dr") file >> config.sensor_rdf_dr;
+        else if (var == "sensor_dir") file >> config.sensor_dir;
         if (file.fail()) return fail("malformed value for key " + var);
     }
     if (components.empty()) return fail("no components defined");
@@ -241,6 +246,79 @@ void TemperatureSensor::measure() {
 
 void VelocityScaling::apply() { check(m_container.ctx(), komd_rescale(m_container.ctx(), m_target), "VelocityScaling"); }
 
+static double box_volume(const Configuration& config) {
+    return (config.domainHigh[0] - config.domainLow[0]) * (config.domainHigh[1] - config.domainLow[1]) *
+           (config.domainHigh[2] - config.domainLow[2]);
+}
+
+static std::string sensor_path(const std::string& dir, const std::string& name) {
+    return dir.empty() ? name : dir + "/" + name;
+}
+
+RDFSensor::RDFSensor(MoleculeContainer& container, const Configuration& config)
+    : m_container(container), m_max_r(config.sensor_rdf_max), m_delta_r(config.sensor_rdf_dr),
+      m_volume(box_volume(config)), m_dir(config.sensor_dir) {
+    const uint64_t num_bins = (uint64_t)(m_max_r / m_delta_r);  // RDFSensor.cpp:19
+    m_bins.assign(num_bins, 0.0);
+    m_sample.assign(num_bins, 0);
+}
+
+void RDFSensor::measure() {
+    check(m_container.ctx(),
+          komd_rdf(m_container.ctx(), m_max_r, m_delta_r, (int64_t)m_sample.size(), m_sample.data()), "RDFSensor");
+    for (size_t k = 0; k < m_bins.size(); k++) m_bins[k] += (double)m_sample[k];
+    m_samples += 1;
+}
+
+// RDFSensor::write (RDFSensor.cpp:37-57)
+void RDFSensor::write(uint64_t simstep) {
+    const std::string name = sensor_path(m_dir, "rdf_" + std::to_string(simstep) + ".txt");
+    std::ofstream file(name);
+    if (!file.is_open()) throw std::runtime_error("Could not write file: " + name);
+    const double n = (double)m_container.getNumMolecules();
+    const double n_total = n * (n - 1);
+    double r = m_delta_r / 2.0;
+    const double dr_half = r;
+    for (size_t idx = 0; idx < m_bins.size(); idx++) {
+        const double n_bin = m_bins[idx] / m_samples;
+        const double v_bin = 4.0 / 3.0 * M_PI * (std::pow(r + dr_half, 3) - std::pow(r - dr_half, 3));
+        const double g_r = (n_bin * m_volume) / (v_bin * n_total);
+        file << r << "\t" << g_r << "\t" << n_bin << "\t" << v_bin << "\t" << n_total << "\t" << m_volume << "\t"
+             << (n_bin * m_volume) << "\t" << (v_bin * n_total) << "\n";
+        r += m_delta_r;
+    }
+}
+
+PressureSensor::PressureSensor(MoleculeContainer& container, const Configuration& config, int kind3b)
+    : m_container(container), m_kind3b(kind3b), m_volume(box_volume(config)), m_dir(config.sensor_dir) {}
+
+void PressureSensor::measure() {
+    komd_observables o{};
+    check(m_container.ctx(), komd_measure_observables(m_container.ctx(), m_kind3b, &o), "PressureSensor");
+    for (int k = 0; k < 6; k++) m_pressures[(size_t)k] = o.kinetic_tensor[k] + o.virial_tensor[k];
+}
+
+// PressureSensor::write (PressureSensor.cpp:34-70): tensor / volume in Pa,
+// factor = conv_Da_kg / (ps_s^2 A_m) = 1.66053906892e-27 / 1e-34.
+void PressureSensor::write(uint64_t simstep) {
+    const std::string name = sensor_path(m_dir, "pressures_" + std::to_string(simstep) + ".txt");
+    std::ofstream file(name);
+    if (!file.is_open()) throw std::runtime_error("Could not write file: " + name);
+    const double factor = 1.66053906892 * std::pow(10.0, 7.0);
+    int next_count = 1, count = 0;
+    for (int idx = 0; idx < 6; idx++) {
+        file << m_pressures[(size_t)idx] / m_volume * factor;
+        count++;
+        if (count == next_count) {
+            count = 0;
+            next_count++;
+            file << std::endl;
+        } else {
+            file << " ";
+        }
+    }
+}
+
 std::unique_ptr<Registry> makeRegistry(const Configuration& config) {
     auto reg = std::make_unique<Registry>();
     reg->config = config;
@@ -268,6 +346,16 @@ std::unique_ptr<Registry> makeRegistry(const Configuration& config) {
     }
     reg->integrators.push_back(std::make_unique<Integrator>(c));
     reg->temperatureSensor = std::make_unique<TemperatureSensor>(c);
+    // sensors in the order main.cpp:81-84 registers them
+    if (config.enable_sensor_rdf) reg->sensors.push_back(std::make_shared<RDFSensor>(c, config));
+    if (config.enable_sensor_pres) {
+        int kind = 0;
+        if (c.threeBody() && !reg->forceFunctors3b.empty()) {
+            if (auto a = std::dynamic_pointer_cast<ATM>(reg->forceFunctors3b[0])) kind = a->kind();
+            if (auto s = std::dynamic_pointer_cast<ATMSurrogate>(reg->forceFunctors3b[0])) kind = s->kind();
+        }
+        reg->sensors.push_back(std::make_shared<PressureSensor>(c, config, kind));
+    }
     if (config.thermostat) reg->thermostats.push_back(std::make_unique<VelocityScaling>(c, config.temperature));
     return reg;
 }
@@ -289,14 +377,20 @@ void Simulation::run(Registry& reg, uint64_t steps) {
         if (c.threeBody())
             for (auto& p : reg.forceFunctors3b) (*p)();
         for (auto& i : reg.integrators) i->integrate1();
+        for (auto& sensor : reg.sensors) sensor->measure();
         reg.temperatureSensor->measure();
         m_simstep++;
+        if (reg.config.write_freq > 0 && m_simstep % reg.config.write_freq == 0)
+            for (auto& sensor : reg.sensors) sensor->write(m_simstep);
         for (auto& t : reg.thermostats) t->apply();
     }
+    for (auto& sensor : reg.sensors) sensor->write(m_simstep);
 }
 
 void Simulation::runDevice(Registry& reg, uint64_t steps) {
     MoleculeContainer& c = *reg.container;
+    if (!reg.sensors.empty())
+        throw std::runtime_error("Simulation::runDevice: sensors measure every step; use the per-functor loop");
     int kind = 0;
     if (c.threeBody() && !reg.forceFunctors3b.empty()) {
         if (auto a = std::dynamic_pointer_cast<ATM>(reg.forceFunctors3b[0])) kind = a->kind();

@@ -9,6 +9,7 @@
 //   MoleculeContainer / LinkedCells  container/MoleculeContainer.h:19-146, LinkedCells.cpp:11-81
 //   Integrator                       integrators/Integrator.h:14-65
 //   TemperatureSensor                sensors/TemperatureSensor.cpp:22-43
+//   Sensor / RDFSensor / PressureSensor  sensors/Sensor.h, RDFSensor.cpp:13-57, PressureSensor.cpp:21-70
 //   Thermostat / VelocityScaling     thermostats/VelocityScaling.cpp:12-25
 //   Simulation::run                  Simulation.cpp:10-80
 //   FileInput::readFile              IO/FileInput.cpp:11-121
@@ -61,6 +62,12 @@ struct Configuration {
     double hybrid_center = 0, hybrid_half = 0, hybrid_blend = 2.0;
     bool thermostat = true;              // velocity rescaling every step (main.cpp:93)
     int checkpoint_precision = 17;       // digits of the .ps writer
+    // sensors (defaults.h:30-46)
+    bool enable_sensor_rdf = false;
+    bool enable_sensor_pres = false;
+    double sensor_rdf_max = 5.0;
+    double sensor_rdf_dr = 0.1;
+    std::string sensor_dir;              // directory of the sensor files (reference: the working directory)
 };
 
 struct Component {  // one COMP line per site: eps (J), sigma, mass, offset
@@ -220,6 +227,52 @@ private:
     double m_temperature = 0;
 };
 
+// Sensor (sensors/Sensor.h): measure() after integrate1 every step, write(simstep) every
+// write_freq steps and at the end (Simulation.cpp:58-79).
+class Sensor {
+public:
+    virtual ~Sensor() = default;
+    virtual void measure() = 0;
+    virtual void write(uint64_t simstep) = 0;
+};
+
+// RDFSensor (RDFSensor.cpp:13-57): ordered-pair counts of molecule COM distances summed
+// over samples; write() prints "rdf_<simstep>.txt" with the reference's eight columns.
+class RDFSensor : public Sensor {
+public:
+    RDFSensor(MoleculeContainer& container, const Configuration& config);
+    void measure() override;
+    void write(uint64_t simstep) override;
+    const std::vector<double>& bins() const { return m_bins; }
+    double samples() const { return m_samples; }
+
+private:
+    MoleculeContainer& m_container;
+    double m_max_r, m_delta_r, m_volume;
+    std::string m_dir;
+    std::vector<double> m_bins;
+    std::vector<uint64_t> m_sample;
+    double m_samples = 0;
+};
+
+// PressureSensor (PressureSensor.cpp:21-70): the stress tensor sum m v v + virial of the
+// registered potentials, XX XY YY XZ YZ ZZ; write() prints "pressures_<simstep>.txt" in Pa,
+// as a lower triangle (one, two, three values per line).
+class PressureSensor : public Sensor {
+public:
+    PressureSensor(MoleculeContainer& container, const Configuration& config, int kind3b);
+    void measure() override;
+    void write(uint64_t simstep) override;
+    const std::array<double, 6>& pressures() const { return m_pressures; }
+
+private:
+    MoleculeContainer& m_container;
+    int m_kind3b;
+    double m_volume;
+    std::string m_dir;
+    std::array<double, 6> m_pressures{};
+};
+
 class Thermostat {
 public:
     virtual ~Thermostat() = default;
@@ -245,6 +298,7 @@ struct Registry {
     std::vector<std::unique_ptr<Integrator>> integrators;
     std::vector<std::unique_ptr<Thermostat>> thermostats;
     std::unique_ptr<TemperatureSensor> temperatureSensor;
+    std::vector<std::shared_ptr<Sensor>> sensors;
 };
 
 // Builds the registry of a deck: container from the checkpoint files, LJ (+ harmonic
@@ -256,7 +310,8 @@ class Simulation {
 public:
     // The reference loop (Simulation.cpp:49-74), one functor call after the other.
     void run(Registry& reg, uint64_t steps);
-    // The same loop queued on the device as one replayed step (komd_run).
+    // The same loop queued on the device as one replayed step (komd_run).  Sensors
+    // measure between functors, so a registry with sensors must use run().
     void runDevice(Registry& reg, uint64_t steps);
     uint64_t simstep() const { return m_simstep; }
 

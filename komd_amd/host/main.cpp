@@ -77,12 +77,12 @@ int main(int argc, char** argv) {
             CheckpointIO::writeCheckpoint(sim.simstep(), *reg->container, config.checkpoint_precision, write_path);
         std::printf(
             "{\"sites\": %zu, \"molecules\": %llu, \"grid\": [%lld, %lld, %lld], \"steps\": %llu, \"loop\": \"%s\", "
-            "\"temperature\": %.17g, \"kinetic\": %.17g, \"u_lj\": %.17g, \"u_3b\": %.17g, \"pairs\": %lld, "
-            "\"triplets\": %lld}\n",
+            "\"temperature\": %.17g, \"kinetic\": %.17g, \"u_lj\": %.17g, \"u_3b\": %.17g, \"virial\": %.17g, "
+            "\"pairs\": %lld, \"triplets\": %lld, \"sensors\": %zu}\n",
             reg->container->sites().size(), (unsigned long long)reg->container->getNumMolecules(), (long long)dims[0],
             (long long)dims[1], (long long)dims[2], (unsigned long long)sim.simstep(),
-            device_loop ? "device" : "reference", o.temperature, o.kinetic, o.u_lj, o.u_3b, (long long)o.pairs,
-            (long long)o.triplets);
+            device_loop ? "device" : "reference", o.temperature, o.kinetic, o.u_lj, o.u_3b, o.virial,
+            (long long)o.pairs, (long long)o.triplets, reg->sensors.size());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "komd_md: %s\n", e.what());
         return 1;

@@ -57,6 +57,8 @@ def lib():
             "or_bonded": (f64, [i64, P, f64, f64, i64, P, f64, f64, P, P, P]),
             "or_run": (i32, [i64, P, P, P, P, P, P, P, P, P, f64, f64, f64, f64, i32, f64, f64, f64, f64, i64, i32]),
             "or_num_threads": (i32, []),
+            "or_virial": (None, [i64, P, P, P, P, f64, P, f64, P, i32, i64, P, f64, f64, i64, P, f64, f64, P]),
+            "or_rdf": (None, [i64, P, P, f64, f64, i64, P, i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -189,6 +191,41 @@ def bonded(sys, r, kb=500.0, r0=1.0, ka=50.0, theta0=np.radians(109.5)):
     u = lib().or_bonded(bonds.shape[0], _p(bonds), float(kb), float(r0), angles.shape[0], _p(angles), float(ka),
                         float(theta0), _p(_c(r, np.float64)), _p(_c(sys.box, np.float64)), _p(f))
     return f, float(u)
+
+
+def virial(sys, r, cutoff=2.5, nu=0.0, site_cell=None, coverage=0, kb=500.0, r0=1.0, ka=50.0,
+           theta0=np.radians(109.5), bonded=True):
+    """Virial tensor (XX XY YY XZ YZ ZZ) of LJ + physical ATM (nu != 0) + harmonic
+    bonds/angles (when the system has them and bonded=True), minimum image."""
+    r = _c(r, np.float64)
+    n = r.shape[0]
+    bonds = _c(sys.bonds if bonded else np.zeros((0, 2)), np.uint32)
+    angles = _c(sys.angles if bonded else np.zeros((0, 3)), np.uint32)
+    sc = _c(site_cell if site_cell is not None else np.zeros(n), np.int64)
+    w = np.zeros(6)
+    lib().or_virial(n, _p(r), _p(_c(sys.mol_id, np.uint64)), _p(_c(sys.sig, np.float64)),
+                    _p(_c(sys.eps, np.float64)), float(cutoff), _p(_c(sys.box, np.float64)), float(nu), _p(sc),
+                    int(coverage), bonds.shape[0], _p(bonds), float(kb), float(r0), angles.shape[0], _p(angles),
+                    float(ka), float(theta0), _p(w))
+    return w
+
+
+def molecule_coms(sys, r):
+    """Centre of mass of every molecule (MoleculeContainer.cpp:55-91), shape (M, 3)."""
+    r = _c(r, np.float64)
+    ms = sys.mol_start()
+    com = np.zeros_like(r)
+    lib().or_update_com(len(ms) - 1, _p(ms), _p(r), _p(_c(sys.mass, np.float64)), _p(com))
+    return com[ms[:-1]].copy()
+
+
+def rdf(com, box, max_r, delta_r, nbins, nthreads=8):
+    """RDFSensor::measure counts over ordered molecule pairs (RDFSensor.cpp:59-72)."""
+    com = _c(com, np.float64)
+    out = np.zeros(int(nbins), np.uint64)
+    lib().or_rdf(com.shape[0], _p(com), _p(_c(box, np.float64)), float(max_r), float(delta_r), int(nbins), _p(out),
+                 int(nthreads))
+    return out
 
 
 def limit(sys, f, limit_factor):

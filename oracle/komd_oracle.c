@@ -22,6 +22,12 @@
  *     rule stated independently (cells all equal or all distinct).
  *   - or_vv_step0/1: harmonic-oscillator closed form, time reversibility.
  *   - or_bonded: finite differences, sum of forces = 0.
+ *   - or_virial: minus the strain derivative of the total energy of an
+ *     isolated cluster, W_ab = -dU/de_ab under r -> (I + e E_ab) r, from the
+ *     brute-force energies (tests/test_oracle_sensors.py).
+ *   - or_rdf: exact coordination shells of a perfect FCC lattice, the count of
+ *     ordered pairs within max_r from a numpy distance matrix, and a numpy
+ *     histogram of the same distances.
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off), so that the
  * FP64 operation order is exactly the one written here (no FMA contraction).
@@ -918,4 +924,171 @@ EXPORT int or_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sensors: virial tensor and radial distribution.                          */
+/* ------------------------------------------------------------------------ */
+
+/* w6 += sym(a (x) b), components XX XY YY XZ YZ ZZ (sensors/PressureSensor.h:18). */
+static inline void sym_add(double* w6, const double* a, const double* b) {
+    w6[0] += a[0] * b[0];
+    w6[1] += 0.5 * (a[0] * b[1] + a[1] * b[0]);
+    w6[2] += a[1] * b[1];
+    w6[3] += 0.5 * (a[0] * b[2] + a[2] * b[0]);
+    w6[4] += 0.5 * (a[1] * b[2] + a[2] * b[1]);
+    w6[5] += a[2] * b[2];
+}
+
+/* Virial tensor W = sum over interactions of sum_members r_m (x) F_m, written as
+ * pair terms of minimum-image relative vectors (periodic-safe):
+ *   LJ pairs of different molecules within cutoff: d (x) F_i, d = r_i - r_j
+ *     (PressureSensor.cpp:152-174);
+ *   ATM triplets within cutoff, physical force (F = -grad V), coverage as
+ *     or_atm_brute: sum over the three pairs d_ab (x) d_ab v_ab with
+ *     v_ab = -comp_force (PressureSensor.cpp:223-287, zz included);
+ *   harmonic bonds d (x) F_j, d = r_j - r_i; angles (r_i - r_j) (x) F_i +
+ *     (r_k - r_j) (x) F_k.
+ * nu = 0 skips the ATM; nb / nang = 0 skip the bonded terms.  Brute-force
+ * neighbour search, O(N^2 + N m^2). */
+EXPORT void or_virial(int64_t n, const double* r, const uint64_t* id, const double* sig, const double* eps,
+                      double cutoff, const double* box, double nu, const int64_t* site_cell, int coverage,
+                      int64_t nb, const uint32_t* bonds, double kb, double r0, int64_t nang, const uint32_t* angles,
+                      double ka, double theta0, double* w6) {
+    const double rc2 = cutoff * cutoff;
+    for (int k = 0; k < 6; k++) w6[k] = 0.0;
+    int64_t cap = 256;
+    int64_t* nbr = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+    double* dv = (double*)malloc(sizeof(double) * 3 * (size_t)cap);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t m = 0;
+        for (int64_t j = i + 1; j < n; j++) {
+            double d[3];
+            mi_disp(r + 3 * i, r + 3 * j, box, d); /* r_j - r_i */
+            double q = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (q > rc2) continue;
+            if (id[i] != id[j]) {
+                double dr[3] = {-d[0], -d[1], -d[2]}; /* r_i - r_j */
+                double invdr2 = 1. / q;
+                double sigma = (sig[i] + sig[j]) / 2.0;
+                double epsilon = sqrt(eps[i] * eps[j]);
+                double lj6 = pow(sigma * sigma * invdr2, 3.0);
+                double lj12 = pow(lj6, 2.0);
+                double fac = 24.0 * epsilon * (2.0 * lj12 - lj6) * invdr2;
+                double fi[3] = {dr[0] * fac, dr[1] * fac, dr[2] * fac};
+                sym_add(w6, dr, fi);
+            }
+            if (m == cap) {
+                cap *= 2;
+                nbr = (int64_t*)realloc(nbr, sizeof(int64_t) * (size_t)cap);
+                dv = (double*)realloc(dv, sizeof(double) * 3 * (size_t)cap);
+            }
+            nbr[m] = j;
+            dv[3 * m] = d[0]; dv[3 * m + 1] = d[1]; dv[3 * m + 2] = d[2];
+            m++;
+        }
+        if (nu == 0.0) continue;
+        for (int64_t a = 0; a < m; a++)
+            for (int64_t b = a + 1; b < m; b++) {
+                const double* dij = dv + 3 * a;
+                const double* dik = dv + 3 * b;
+                double djk[3] = {dik[0] - dij[0], dik[1] - dij[1], dik[2] - dij[2]};
+                double qjk = djk[0] * djk[0] + djk[1] * djk[1] + djk[2] * djk[2];
+                if (qjk > rc2) continue;
+                if (coverage == 1) {
+                    int64_t ci = site_cell[i], cj = site_cell[nbr[a]], ck = site_cell[nbr[b]];
+                    int same = (ci == cj) && (cj == ck);
+                    int distinct = (ci != cj) && (cj != ck) && (ci != ck);
+                    if (!same && !distinct) continue;
+                }
+                double qij = dij[0] * dij[0] + dij[1] * dij[1] + dij[2] * dij[2];
+                double qik = dik[0] * dik[0] + dik[1] * dik[1] + dik[2] * dik[2];
+                double ra = sqrt(qij), rb = sqrt(qik), rcn = sqrt(qjk);
+                double v01 = -or_atm_comp_force(nu, ra, rb, rcn);
+                double v02 = -or_atm_comp_force(nu, rb, ra, rcn);
+                double v12 = -or_atm_comp_force(nu, rcn, ra, rb);
+                double d01[3], d02[3], d12[3], f01[3], f02[3], f12[3];
+                for (int q = 0; q < 3; q++) {
+                    d01[q] = -dij[q]; d02[q] = -dik[q]; d12[q] = -djk[q];
+                    f01[q] = d01[q] * v01; f02[q] = d02[q] * v02; f12[q] = d12[q] * v12;
+                }
+                sym_add(w6, d01, f01);
+                sym_add(w6, d02, f02);
+                sym_add(w6, d12, f12);
+            }
+    }
+    free(nbr);
+    free(dv);
+    for (int64_t t = 0; t < nb; t++) {
+        uint32_t i = bonds[2 * t], j = bonds[2 * t + 1];
+        double d[3];
+        mi_disp(r + 3 * i, r + 3 * j, box, d);
+        double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double g = -kb * (len - r0) / len;
+        double fj[3] = {g * d[0], g * d[1], g * d[2]};
+        sym_add(w6, d, fj);
+    }
+    for (int64_t t = 0; t < nang; t++) {
+        uint32_t i = angles[3 * t], j = angles[3 * t + 1], k = angles[3 * t + 2];
+        double e1[3], e2[3];
+        mi_disp(r + 3 * j, r + 3 * i, box, e1);
+        mi_disp(r + 3 * j, r + 3 * k, box, e2);
+        double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        double c = (e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2]) / (l1 * l2);
+        if (c > 1) c = 1;
+        if (c < -1) c = -1;
+        double dth = acos(c) - theta0;
+        double s = sqrt(1 - c * c);
+        if (s < 1e-12) s = 1e-12;
+        double pre = ka * dth / s;
+        double fi[3], fk[3];
+        for (int q = 0; q < 3; q++) {
+            fi[q] = pre * (e2[q] / (l1 * l2) - c * e1[q] / (l1 * l1));
+            fk[q] = pre * (e1[q] / (l1 * l2) - c * e2[q] / (l2 * l2));
+        }
+        sym_add(w6, e1, fi);
+        sym_add(w6, e2, fk);
+    }
+}
+
+/* RDFSensor::RDF_Kernel over every ordered pair of distinct molecules
+ * (RDFSensor.cpp:59-72): delta = c0 - c1, dr = delta - L round(delta / L) per
+ * axis, r = sqrt(x^2 + y^2 + z^2) summed in that order (Array::L2), skipped
+ * when r > max_r, else one count in clamp((uint64)(r / dr), 0, nbins - 1)
+ * (getBin, RDFSensor.h:43-48).  com: molecule centres, 3 per molecule. */
+EXPORT void or_rdf(int64_t m, const double* com, const double* box, double max_r, double delta_r, int64_t nbins,
+                   uint64_t* counts, int nthreads) {
+    for (int64_t b = 0; b < nbins; b++) counts[b] = 0;
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        uint64_t* loc = (uint64_t*)calloc((size_t)nbins, sizeof(uint64_t));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t i = 0; i < m; i++)
+            for (int64_t j = 0; j < m; j++) {
+                if (i == j) continue;
+                double d[3];
+                for (int q = 0; q < 3; q++) {
+                    double delta = com[3 * i + q] - com[3 * j + q];
+                    d[q] = delta - box[q] * round(delta / box[q]);
+                }
+                double s = 0;
+                for (int q = 0; q < 3; q++) s += d[q] * d[q];
+                double r = sqrt(s);
+                if (r > max_r) continue;
+                uint64_t bin = (uint64_t)(r / delta_r);
+                if (bin > (uint64_t)(nbins - 1)) bin = (uint64_t)(nbins - 1);
+                loc[bin]++;
+            }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        for (int64_t b = 0; b < nbins; b++) counts[b] += loc[b];
+        free(loc);
+    }
 }
