@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--no-hybrid", action="store_true", help="skip the 1,048,576-atom hybrid slab leg")
     ap.add_argument("--no-other", action="store_true", help="skip the LJ-4000 and chains-262144 legs")
     ap.add_argument("--hybrid-steps", type=int, default=100)
+    ap.add_argument("--structure-steps", type=int, default=1000,
+                    help="NVE steps of the g(r) / pressure comparison, surrogate against exact (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -90,6 +92,23 @@ def drift_of(ctx, K, n, kind, steps):
             ctx.run(chunk, kind3b=kind)
     return {"steps": 10 * chunk, "slope_e_per_atom_per_tau": float(np.polyfit(ts, es, 1)[0]),
             "max_dev_per_atom": float(np.max(np.abs(np.array(es) - es[0])))}
+
+
+def structure_of(ctx, K, kind, n, vol, steps, every, max_r=4.0, dr=0.05):
+    """g(r) (RDFSensor counts normalised as RDFSensor::write) and the scalar pressure
+    (kinetic + virial tensor trace / 3V) sampled every `every` NVE steps."""
+    nb = K.rdf_bins(max_r, dr)
+    acc = np.zeros(nb)
+    ps = []
+    ctx.compute_forces(kind)
+    for _ in range(steps // every):
+        ctx.run(every, kind3b=kind)
+        acc += ctx.rdf(max_r, dr)
+        o = ctx.observables(kind)
+        t = o["kinetic_tensor"] + o["virial_tensor"]
+        ps.append((t[0] + t[2] + t[5]) / (3.0 * vol))
+    r, g = K.rdf_normalise(acc, steps // every, n, vol, dr)
+    return r, g, np.array(ps)
 
 
 def learned_leg(n_atoms, args):
@@ -147,6 +166,30 @@ def learned_leg(n_atoms, args):
             ctx.set_state(r0, v0)
             ctx.compute_forces(K.SURROGATE)
             out["energy_drift_surrogate"] = drift_of(ctx, K, n, K.SURROGATE, args.drift_steps)
+        if n_atoms == 32000 and args.structure_steps > 0:
+            # structure and pressure of the surrogate's trajectory against the exact one's,
+            # both from the same equilibrated state
+            vol = float(np.prod(sysm.box))
+            st = {}
+            for kind, name in ((K.ATM_EXACT, "exact"), (K.SURROGATE, "surrogate")):
+                ctx.set_state(r0, v0)
+                st[name] = structure_of(ctx, K, kind, n, vol, args.structure_steps, 10)
+            r_, g_e, p_e = st["exact"]
+            _, g_s, p_s = st["surrogate"]
+            k1 = int(np.argmax(g_e))
+            ctx.timing(True)
+            ctx.rdf(4.0, 0.05)
+            rdf_ms, rdf_n = ctx.timer("rdf")
+            ctx.timing(False)
+            out["structure"] = {
+                "samples": args.structure_steps // 10, "steps": args.structure_steps,
+                "rdf": "max_r 4.0, dr 0.05, molecule COMs, ordered pairs (RDFSensor.cpp:59-72)",
+                "g_rms_difference": float(np.sqrt(np.mean((g_s - g_e) ** 2))),
+                "g_max_difference": float(np.abs(g_s - g_e).max()),
+                "first_peak": {"r": float(r_[k1]), "g_exact": float(g_e[k1]), "g_surrogate": float(g_s[k1])},
+                "pressure_reduced": {"exact": [float(p_e.mean()), float(p_e.std())],
+                                     "surrogate": [float(p_s.mean()), float(p_s.std())]},
+                "rdf_sample_us": round(rdf_ms / max(rdf_n, 1) * 1e3, 1)}
     return out
 
 

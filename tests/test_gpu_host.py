@@ -118,3 +118,39 @@ def test_reference_checkpoint_lj_forces(orc):
     dd = np.linalg.norm(f - f_or, axis=1)
     print(f"reference checkpoint LJ: max {dd.max() / scale:.2e} rms {np.sqrt(np.mean(dd ** 2)) / scale:.2e}")
     assert dd.max() / scale < 2e-4 and np.sqrt(np.mean(dd ** 2)) / scale < 5e-6
+
+
+def test_sensor_files(orc, tmp_path):
+    """RDFSensor and PressureSensor of komd_md (deck keys enable_sensor_rdf/_pres): one
+    measured step, files written at the end (RDFSensor.cpp:37-57, PressureSensor.cpp:34-70),
+    checked against the oracle's RDF counts and the library's stress tensor on the state
+    komd_md wrote."""
+    s = S.fcc(6, 0.8, 0.8, 74, jitter=0.05)
+    out = tmp_path / "sensors"
+    out.mkdir()
+    d = deck_for(tmp_path, s, 1, extra=(f"thermostat 0\nwrite_freq 100\nenable_sensor_rdf 1\nsensor_rdf_max 4.0\n"
+                                        f"sensor_rdf_dr 0.1\nenable_sensor_pres 1\nsensor_dir {out}\n"))
+    end = str(tmp_path / "end.ps")
+    info = run_md(d, "--write", end)
+    assert info["sensors"] == 2
+    a = np.loadtxt(end)
+    r = a[:, 1:4].copy()
+    vol = float(np.prod(s.box))
+    # RDF: one sample of ordered-pair counts -> g(r) as the reference writes it
+    cnt = orc.rdf(orc.molecule_coms(s, r), s.box, 4.0, 0.1, 40)
+    rr, g_ref = K.rdf_normalise(cnt, 1, s.n, vol, 0.1)
+    rdf = np.loadtxt(out / "rdf_1.txt")
+    assert rdf.shape == (40, 8)
+    assert np.allclose(rdf[:, 0], rr, rtol=1e-5)
+    assert np.allclose(rdf[:, 1], g_ref, rtol=2e-5, atol=1e-12)
+    assert np.allclose(rdf[:, 2], cnt, rtol=2e-5)
+    # pressure: (kinetic + virial tensor) / V in Pa, lower triangle XX / XY YY / XZ YZ ZZ
+    lines = (out / "pressures_1.txt").read_text().split("\n")
+    vals = np.array([float(x) for ln in lines[:3] for x in ln.split()])
+    assert [len(ln.split()) for ln in lines[:3]] == [1, 2, 3]
+    s2 = s.copy()
+    s2.r, s2.v = r, a[:, 4:7].copy()
+    with K.Context(s2, nu=NU) as c:
+        o = c.observables(K.ATM_REF)
+    ref = (o["kinetic_tensor"] + o["virial_tensor"]) / vol * 1.66053906892e7
+    assert np.allclose(vals, ref, rtol=2e-5, atol=2e-5 * np.abs(ref).max())
