@@ -60,7 +60,7 @@ struct CellArgs {
     double* __restrict__ fz;
     float4* __restrict__ nl;       // [N][kFullCap] neighbour list of every site (cell-sorted order)
     uint32_t* __restrict__ nlc;    // [N] its length
-    uint8_t* __restrict__ nld;     // [N][kFullCap] neighbour cell direction (reference-mode ATM)
+    uint16_t* __restrict__ nld;    // [N][kFullCap] neighbour cell direction | rank in that cell << 5
     double* __restrict__ part;
     double* __restrict__ vpart;    // [kSym][nblocks] virial tensor partials (energy mode)
     uint32_t* __restrict__ flags;
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     constexpr int kStage = kChunkLJ;
     constexpr int kWaves = kCellThreads / 64;
     __shared__ float4 pos[kStage];
-    __shared__ uint8_t sdir[NL == 2 ? kStage : 1];
+    __shared__ uint16_t sdir[NL == 2 ? kStage : 1];
     __shared__ float2 cprm[UNIFORM ? 1 : kStage];
     __shared__ uint32_t cmol[MOLS ? kStage : 1];
     __shared__ int nb_start[27];
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                                      __uint_as_float(__float_as_uint(v.w) | nb_code[k]));
                 if (!UNIFORM) cprm[t] = a.sprm[j];
                 if (MOLS) cmol[t] = a.sid[j];
-                if (NL == 2) sdir[t] = (uint8_t)k;
+                if (NL == 2) sdir[t] = (uint16_t)(k | ((gidx - nb_pref[k]) << 5));
             }
             __syncthreads();
             // every lane of the workgroup runs the same trip count (ballots need the whole wave)
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
 struct AtmArgs {
     const float4* __restrict__ nl;
     const uint32_t* __restrict__ nlc;
-    const uint8_t* __restrict__ nld;
+    const uint16_t* __restrict__ nld;
     const uint32_t* __restrict__ sorted;
     const double* __restrict__ rx;
     const double* __restrict__ ry;
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             const float4 v = row[s];
             wbuf[wvu][s] = v;
             if (s == lane) xa = v;
-            if (ATM == 2) wdir[wvu][s] = a.nld[(size_t)p * kFullCap + s];
+            if (ATM == 2) wdir[wvu][s] = (uint8_t)(a.nld[(size_t)p * kFullCap + s] & 31u);
             if (HYB) {
                 const float lj = a.lam[__float_as_uint(v.w) & kIdxMask];
                 wlam[wvu][s] = lj;

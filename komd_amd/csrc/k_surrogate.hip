@@ -180,6 +180,9 @@ __device__ __forceinline__ void radial_gauss(float y, float g[NR]) {
 // lanes per site of the per-site kernels (neighbours strided over the lanes, sums
 // combined with shuffles inside the lane group)
 constexpr int kLps = 4;
+// Sites of the 27-cell neighbourhood held by the surrogate force pass (about 394 at
+// density 0.8; a larger neighbourhood is reported as a cell overflow).
+constexpr int kSurM = 640;
 
 // ---------------------------------------------------------------------------
 // descriptors
@@ -194,6 +197,29 @@ struct DescArgs {
     int n;
     float pirc;
 };
+
+// Radial window: the kWin Gaussians around y.  Every Gaussian outside
+// k0 .. k0 + kWin - 1, k0 = clamp(floor(y) - 4, 0, NR - kWin), lies at |y - k| >= 4,
+// i.e. below exp(-16) = 1.1e-7, under the FP32 resolution of the sums; the descriptor
+// and force passes skip them (the training kernels keep all 32).
+constexpr int kWin = 9;
+__device__ __forceinline__ int win_start(float y) {
+    const int k = (int)floorf(y) - 4;
+    return k < 0 ? 0 : (k > NR - kWin ? NR - kWin : k);
+}
+// g[t] = exp(-(e - t)^2), t < kWin, e = y - k0: one exponential for g[0], one for the
+// ratio g[t+1] / g[t] = exp(2(e - t) - 1), which falls by e^-2 per step.
+__device__ __forceinline__ void win_gauss(float e, float g[kWin]) {
+    float v = __expf(-e * e);
+    float R = __expf(2.f * e - 1.f);
+    g[0] = v;
+#pragma unroll
+    for (int t = 1; t < kWin; t++) {
+        v *= R;
+        R *= 0.1353352832366127f;  // e^-2
+        g[t] = v;
+    }
+}
 
 __global__ __launch_bounds__(256) void k_sur_desc(DescArgs a) {
     const int tid = blockIdx.x * 256 + threadIdx.x;
@@ -529,7 +555,7 @@ enum { FORCE_APPLY = 0, FORCE_TRAIN = 1, FORCE_OUT = 2 };
 struct ForceArgs {
     const float4* __restrict__ nl;
     const uint32_t* __restrict__ nlc;
-    const uint8_t* __restrict__ nld;
+    const uint16_t* __restrict__ nld;
     const uint32_t* __restrict__ inv;
     const uint32_t* __restrict__ sorted;
     const uint32_t* __restrict__ key;
@@ -560,13 +586,19 @@ __device__ __forceinline__ int wrap_cell(const Grid& g, int x, int y, int z) {
     return x + g.nx * (y + g.ny * z);
 }
 
+// lps lanes per home site (16, 8 or 4 by the cell's occupancy) stride over the site's
+// list; each lane holds the site's angular state in registers and reads the radial
+// window of a pair from the staged state in LDS.  c = dE_i/dd_ij: +c is summed in
+// registers and combined inside the lane group once per site, -c goes to j's slot of
+// the LDS accumulator (cell direction and rank carried by the list entry).
 // VIR: also the virial sum over directed pairs of -(r_j - r_i) (x) c (c is the force on i
 // from E_i through the pair; -c acts on j), per-block partials in a.vpart.
+constexpr int kSurForceThreads = 256;
 template <bool VIR>
-__global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
+__global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
     __shared__ int nb_start[27], nb_cnt[27], nb_base[28];
-    __shared__ float acc[3][kMaxM];
-    __shared__ float hst[kRows * NS];
+    __shared__ float acc[3][kSurM];
+    __shared__ float4 hst4[kRows * NS / 4];
     __shared__ int bad;
     const int tid = threadIdx.x, b = blockIdx.x;
     const int cell = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous slab of cells per XCD
@@ -588,7 +620,7 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
             nb_base[k + 1] = nb_base[k] + nb_cnt[k];
             if (nb_cnt[k] > kRows) ok = 0;
         }
-        if (nb_base[27] > kMaxM) ok = 0;
+        if (nb_base[27] > kSurM) ok = 0;
         bad = !ok;
         if (!ok) atomicOr(a.flags, FLAG_CELL_OVERFLOW);
     }
@@ -599,79 +631,87 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
     }
     const int m = nb_base[27], hs = nb_start[13], nh = nb_cnt[13], self = nb_base[13];
     float wv[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int e = tid; e < m; e += 256) acc[0][e] = acc[1][e] = acc[2][e] = 0.f;
+    for (int e = tid; e < m; e += kSurForceThreads) acc[0][e] = acc[1][e] = acc[2][e] = 0.f;
     {
         const float4* src = reinterpret_cast<const float4*>(a.S + (size_t)hs * NS);
-        float4* dst = reinterpret_cast<float4*>(hst);
-        for (int e = tid; e < nh * (NS / 4); e += 256) dst[e] = src[e];
+        for (int e = tid; e < nh * (NS / 4); e += kSurForceThreads) hst4[e] = src[e];
     }
     __syncthreads();
+    const float* hst = reinterpret_cast<const float*>(hst4);
     const int lps = nh <= 16 ? 16 : (nh <= 32 ? 8 : 4);
-    const int h = tid / lps, sub = tid % lps;
-    if (h < nh) {
-        const float* si = hst + h * NS;
-        // sum_k G_k dphi_k = dfc A - 2 fc/w (y A - B), A = sum G_k g_k, B = sum k G_k g_k
-        float Gi[NR], Gk[NR], Ai[80];
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-            Gi[k] = si[k];
-            Gk[k] = (float)k * si[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 80; k++) Ai[k] = si[NR + k];
-        const int p = hs + h;
-        int nn = (int)a.nlc[p];
-        if (a.dlam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // hybrid: no surrogate energy on this site
-        const float4* row = a.nl + (size_t)p * kFullCap;
-        const uint8_t* drow = a.nld + (size_t)p * kFullCap;
+    const int per = kSurForceThreads / lps;
+    for (int h0 = 0; h0 < nh; h0 += per) {
+        const int h = h0 + tid / lps, sub = tid % lps;
+        const bool live = h < nh;
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        // the next entry and its direction are loaded one iteration ahead; the neighbour's
-        // cell slot is needed only after the arithmetic
-        float4 vnext = sub < nn ? row[sub] : make_float4(1.f, 0.f, 0.f, 0.f);
-        int dnext = sub < nn ? (int)drow[sub] : 13;
-        for (int s = sub; s < nn; s += lps) {
-            const float4 v = vnext;
-            const int d = dnext;
-            if (s + lps < nn) {
-                vnext = row[s + lps];
-                dnext = drow[s + lps];
-            }
-            const int pjn = (int)a.inv[__float_as_uint(v.w) & kIdxMask];
-            const Pair q = pair_of(v, a.pirc);
-            const float y = (q.r - kRMu0) * kRInvW;
-            float g[NR];
-            radial_gauss(y, g);
-            float sa = 0.f, sb = 0.f;
+        if (live) {
+            const float* si = hst + h * NS;
+            float A[80];
+            {
+                const float4* ar = reinterpret_cast<const float4*>(si + NR);
 #pragma unroll
-            for (int k = 0; k < NR; k++) {
-                sa += Gi[k] * g[k];
-                sb += Gk[k] * g[k];
+                for (int k = 0; k < 20; k++) {
+                    const float4 t4 = ar[k];
+                    A[4 * k] = t4.x; A[4 * k + 1] = t4.y; A[4 * k + 2] = t4.z; A[4 * k + 3] = t4.w;
+                }
             }
-            const float coef = q.dfc * sa - 2.f * kRInvW * q.fc * (y * sa - sb);
-            float cx = coef * q.ux, cy = coef * q.uy, cz = coef * q.uz;
+            const int p = hs + h;
+            int nn = (int)a.nlc[p];
+            if (a.dlam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // hybrid: no surrogate energy on this site
+            const float4* row = a.nl + (size_t)p * kFullCap;
+            const uint16_t* drow = a.nld + (size_t)p * kFullCap;
+            // the next entry is loaded one iteration ahead of its use
+            float4 vnext = sub < nn ? row[sub] : make_float4(1.f, 0.f, 0.f, 0.f);
+            uint32_t dnext = sub < nn ? drow[sub] : 0u;
+            for (int s = sub; s < nn; s += lps) {
+                const float4 v = vnext;
+                const uint32_t dr = dnext;
+                if (s + lps < nn) {
+                    vnext = row[s + lps];
+                    dnext = drow[s + lps];
+                }
+                const Pair q = pair_of(v, a.pirc);
+                // radial: sum_k S_k d(G_k fc)/dr = dfc sa - 2 fc / w ((y - k0) sa - sc),
+                // sa = sum_t S_{k0+t} g_t, sc = sum_t t S_{k0+t} g_t
+                const float y = (q.r - kRMu0) * kRInvW;
+                const int k0 = win_start(y);
+                const float ew = y - (float)k0;
+                float g[kWin];
+                win_gauss(ew, g);
+                float sa = 0.f, sc = 0.f;
 #pragma unroll
-            for (int ch = 0; ch < NA; ch++) {
-                float psi, dpsi;
-                ang_basis(q, ch, psi, dpsi);
-                chan_grad(Ai + 20 * ch, q.ux, q.uy, q.uz, dpsi, psi * q.ir, cx, cy, cz);
+                for (int t = 0; t < kWin; t++) {
+                    const float w = si[k0 + t] * g[t];
+                    sa += w;
+                    sc += (float)t * w;
+                }
+                const float coef = q.dfc * sa - 2.f * kRInvW * q.fc * (ew * sa - sc);
+                float cx = coef * q.ux, cy = coef * q.uy, cz = coef * q.uz;
+#pragma unroll
+                for (int ch = 0; ch < NA; ch++) {
+                    float psi, dpsi;
+                    ang_basis(q, ch, psi, dpsi);
+                    chan_grad(A + 20 * ch, q.ux, q.uy, q.uz, dpsi, psi * q.ir, cx, cy, cz);
+                }
+                fx += cx;
+                fy += cy;
+                fz += cz;
+                if (VIR) sym_outer<float>(wv, -v.x, -v.y, -v.z, cx, cy, cz);
+                const int d = (int)(dr & 31u), rk = (int)(dr >> 5);
+                if (rk < nb_cnt[d]) {  // lists and cells come from one binning
+                    const int slot = nb_base[d] + rk;
+                    atomicAdd(&acc[0][slot], -cx);
+                    atomicAdd(&acc[1][slot], -cy);
+                    atomicAdd(&acc[2][slot], -cz);
+                }
             }
-            fx += cx;
-            fy += cy;
-            fz += cz;
-            if (VIR) sym_outer<float>(wv, -v.x, -v.y, -v.z, cx, cy, cz);
-            const int rk = pjn - nb_start[d];
-            if ((unsigned)rk >= (unsigned)nb_cnt[d]) continue;  // lists and cells come from one binning
-            const int slot = nb_base[d] + rk;
-            atomicAdd(&acc[0][slot], -cx);
-            atomicAdd(&acc[1][slot], -cy);
-            atomicAdd(&acc[2][slot], -cz);
         }
         for (int o = 1; o < lps; o <<= 1) {
             fx += __shfl_xor(fx, o);
             fy += __shfl_xor(fy, o);
             fz += __shfl_xor(fz, o);
         }
-        if (sub == 0) {
+        if (live && sub == 0) {
             atomicAdd(&acc[0][self + h], fx);
             atomicAdd(&acc[1][self + h], fy);
             atomicAdd(&acc[2][self + h], fz);
@@ -680,7 +720,7 @@ __global__ __launch_bounds__(256) void k_sur_force(ForceArgs a) {
     __syncthreads();
     float4* out = a.P + (size_t)cell * 27 * kRows;
     for (int d = 0; d < 27; d++)
-        for (int r = tid; r < nb_cnt[d]; r += 256) {
+        for (int r = tid; r < nb_cnt[d]; r += kSurForceThreads) {
             const int e = nb_base[d] + r;
             out[d * kRows + r] = make_float4(acc[0][e], acc[1][e], acc[2][e], 0.f);
         }
@@ -1278,9 +1318,9 @@ hipError_t surrogate_pass(komd_ctx* c, bool energy, bool hybrid, int mode) {
         if (energy) {
             a.vpart = c->d.vpart;
             a.vparts = surrogate_virial_blocks(c);
-            k_sur_force<true><<<8 * a.chunk, 256, 0, c->stream>>>(a);
+            k_sur_force<true><<<8 * a.chunk, kSurForceThreads, 0, c->stream>>>(a);
         } else {
-            k_sur_force<false><<<8 * a.chunk, 256, 0, c->stream>>>(a);
+            k_sur_force<false><<<8 * a.chunk, kSurForceThreads, 0, c->stream>>>(a);
         }
         KOMD_CHECK_HIP(hipGetLastError());
     }
@@ -1446,7 +1486,7 @@ hipError_t train_step(komd_ctx* c, int64_t k, float wE, float wF, float lr, bool
         a.part = s->lpart + kTrainBlocks;
         a.cF = 2.f * wF / (3.f * (float)n);
         a.lF = wF / (3.f * (float)n);
-        k_sur_force<false><<<8 * a.chunk, 256, 0, c->stream>>>(a);
+        k_sur_force<false><<<8 * a.chunk, kSurForceThreads, 0, c->stream>>>(a);
         KOMD_CHECK_HIP(hipGetLastError());
         k_sur_apply<FORCE_TRAIN><<<fb, 256, 0, c->stream>>>(a);
         KOMD_CHECK_HIP(hipGetLastError());

@@ -63,7 +63,7 @@ struct DevState {
     // full neighbour lists (three-body and surrogate passes)
     float4* nl = nullptr;
     uint32_t* nlc = nullptr;
-    uint8_t* nld = nullptr;
+    uint16_t* nld = nullptr;      // [N][kFullCap] neighbour cell direction | rank in that cell << 5
     // bonded
     uint32_t *bonds = nullptr, *angles = nullptr;
     uint32_t *bond_mol = nullptr, *angle_mol = nullptr;  // CSR of bonds / angles per molecule
