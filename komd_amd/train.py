@@ -84,10 +84,10 @@ def main(argv=None):
     ap.add_argument("--equil", type=int, default=1000, help="melting steps at the top temperature")
     ap.add_argument("--tmin", type=float, default=0.7)
     ap.add_argument("--tmax", type=float, default=1.0)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr0", type=float, default=2e-3)
     ap.add_argument("--lr1", type=float, default=5e-5)
-    ap.add_argument("--lam-e", type=float, default=0.1)
+    ap.add_argument("--lam-e", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=1004)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--log", default=None)
@@ -110,7 +110,7 @@ def main(argv=None):
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         np.save(a.out, params)
         info.update({"atoms": sysm.n, "snapshots": a.snapshots, "every": a.every, "t_range": [a.tmin, a.tmax],
-                     "epochs": a.epochs, "seed": a.seed})
+                     "epochs": a.epochs, "lam_e": a.lam_e, "seed": a.seed})
         with open(os.path.splitext(a.out)[0] + ".json", "w") as fh:
             json.dump(info, fh, indent=1)
         log(f"saved {a.out}")
