@@ -339,6 +339,7 @@ constexpr int kAtmWaves = 4;
 template <int ATM, bool ENERGY, bool HYB>
 __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     __shared__ float4 wbuf[kAtmWaves][kFullCap];
+    __shared__ float2 wsq[kAtmWaves][kFullCap];  // |x_a|^2 and its reciprocal per neighbour
     __shared__ uint32_t qbuf[kAtmWaves][128];
     __shared__ uint8_t wdir[ATM == 2 ? kAtmWaves : 1][ATM == 2 ? kFullCap : 1];
     __shared__ float wlam[HYB ? kAtmWaves : 1][HYB ? kFullCap : 1];
@@ -365,6 +366,8 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
         for (int s = lane; s < nn; s += 64) {
             const float4 v = row[s];
             wbuf[wvu][s] = v;
+            const float X = v.x * v.x + v.y * v.y + v.z * v.z;
+            wsq[wvu][s] = make_float2(X, __builtin_amdgcn_rcpf(X));
             if (s == lane) xa = v;
             if (ATM == 2) wdir[wvu][s] = (uint8_t)(a.nld[(size_t)p * kFullCap + s] & 31u);
             if (HYB) {
@@ -378,6 +381,9 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
         if (HYB && __ballot(weighted) == 0ull) p_skip = true;
         if (!p_skip) {
             uint32_t* ring = qbuf[wvu];
+            // the visit energy is needed by the energy passes, the hybrid weight gradient
+            // and the per-site energy output only
+            const bool want_e = ENERGY || HYB || a.esite != nullptr;
             float fx = 0.f, fy = 0.f, fz = 0.f, ev = 0.f;
             // dense evaluation of an in-range pair (x = r_j - r_i, y = r_k - r_i)
             auto force = [&](int ia, int ib) {
@@ -385,8 +391,8 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
                 float wt = 1.f;
                 if (HYB) wt = (li + wlam[wvu][ia] + wlam[wvu][ib]) * (1.f / 3.f);
-                const float X = xv.x * xv.x + xv.y * xv.y + xv.z * xv.z;
-                const float Y = yv.x * yv.x + yv.y * yv.y + yv.z * yv.z;
+                const float2 sx = wsq[wvu][ia], sy = wsq[wvu][ib];
+                const float X = sx.x, Y = sy.x;
                 const float Z = zx * zx + zy * zy + zz * zz;
                 const float p1 = xv.x * yv.x + xv.y * yv.y + xv.z * yv.z;
                 const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
@@ -399,7 +405,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 float K = 3.f * a.nu * D52 * wt;
                 if (ATM == 2) K = -K;  // reference: +grad V
                 const float A = -(D + 5.f * Pp);
-                const float iX = __builtin_amdgcn_rcpf(X), iY = __builtin_amdgcn_rcpf(Y);
+                const float iX = sx.y, iY = sy.y;
                 const float t1 = p2 * p3, t23 = p1 * (p3 + p2);
                 const float gx = K * (A * (xv.x * iX + yv.x * iY) - t1 * (xv.x + yv.x) - t23 * zx);
                 const float gy = K * (A * (xv.y * iX + yv.y * iY) - t1 * (xv.y + yv.y) - t23 * zy);
@@ -407,7 +413,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 fx += gx;
                 fy += gy;
                 fz += gz;
-                ev += a.nu * D52 * (D + 3.f * Pp);  // unweighted V of this visit
+                if (want_e) ev += a.nu * D52 * (D + 3.f * Pp);  // unweighted V of this visit
                 if (ENERGY) {
                     // (r_i - centroid) (x) F_i, r_i - centroid = -(x + y)/3; the three member
                     // visits sum to sum_m r_m (x) F_m.  Physical sign in both modes.
@@ -452,7 +458,8 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                     yi = __builtin_amdgcn_update_dpp(0, yi, 0x134, 0xF, 0xF, false);
                     const float zx = yx - xa.x, zy = yy - xa.y, zz = yz - xa.z;
                     const float Z = zx * zx + zy * zy + zz * zz;
-                    const bool valid = lane < nn && yi < nn && (k < 32 || lane < 32);
+                    // lanes past the list hold distinct far-away fillers: no pair with them is in range
+                    const bool valid = k < 32 || lane < 32;
                     bool in = false;
                     if (valid) {
                         if (fabsf(Z - a.rc2f) <= a.band3)
@@ -500,7 +507,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             }
             nvisit = (unsigned long long)tail;
             double dx = wave_sum((double)fx), dy = wave_sum((double)fy), dz = wave_sum((double)fz);
-            const double de = wave_sum((double)ev) * (1.0 / 3.0);  // this site's third of its triplets
+            const double de = want_e ? wave_sum((double)ev) * (1.0 / 3.0) : 0.0;  // this site's third of its triplets
             if (lane == 0) {
                 if (HYB) {  // -V grad(w): site i receives -(sum V/3) dl_i/dx_axis
                     const double t = -de * (double)a.dlam[oi];
