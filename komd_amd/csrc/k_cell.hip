@@ -451,11 +451,13 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 // lane a holds y = x_{a-k mod 64} and its lane index yi
                 float yx = xa.x, yy = xa.y, yz = xa.z;
                 int yi = lane;
+                // wave_rol has a source lane for every lane: plain DPP moves, no old value
+#pragma unroll 2
                 for (int k = 1; k <= 32; k++) {
-                    yx = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yx), 0x134, 0xF, 0xF, false));
-                    yy = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yy), 0x134, 0xF, 0xF, false));
-                    yz = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(yz), 0x134, 0xF, 0xF, false));
-                    yi = __builtin_amdgcn_update_dpp(0, yi, 0x134, 0xF, 0xF, false);
+                    yx = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yx), 0x134, 0xF, 0xF, false));
+                    yy = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yy), 0x134, 0xF, 0xF, false));
+                    yz = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yz), 0x134, 0xF, 0xF, false));
+                    yi = __builtin_amdgcn_mov_dpp(yi, 0x134, 0xF, 0xF, false);
                     const float zx = yx - xa.x, zy = yy - xa.y, zz = yz - xa.z;
                     const float Z = zx * zx + zy * zy + zz * zz;
                     // lanes past the list hold distinct far-away fillers: no pair with them is in range
