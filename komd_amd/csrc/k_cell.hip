@@ -340,7 +340,7 @@ template <int ATM, bool ENERGY, bool HYB>
 __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     __shared__ float4 wbuf[kAtmWaves][kFullCap];
     __shared__ float2 wsq[kAtmWaves][kFullCap];  // |x_a|^2 and its reciprocal per neighbour
-    __shared__ uint32_t qbuf[kAtmWaves][128];
+    __shared__ uint32_t qbuf[kAtmWaves][128 + 64];  // ring of 128 pairs + one spill slot per lane
     __shared__ uint8_t wdir[ATM == 2 ? kAtmWaves : 1][ATM == 2 ? kFullCap : 1];
     __shared__ float wlam[HYB ? kAtmWaves : 1][HYB ? kFullCap : 1];
     __shared__ double red[kAtmWaves];
@@ -421,12 +421,14 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                     sym_outer<float>(vir, sc * (xv.x + yv.x), sc * (xv.y + yv.y), sc * (xv.z + yv.z), gx, gy, gz);
                 }
             };
-            const unsigned long long below = (1ull << lane) - 1ull;
             int head = 0, tail = 0;
             // ballot-compact this instruction's in-range pairs into the ring; dense flush at 64
+            // (lanes without a pair write their own spill slot, so the store needs no branch)
             auto push = [&](bool hit, int ia, int ib) {
                 const unsigned long long bal = __ballot(hit);
-                if (hit) ring[(tail + __popcll(bal & below)) & 127] = (uint32_t)ia | ((uint32_t)ib << 16);
+                const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                ring[hit ? ((tail + pos) & 127) : 128 + lane] = (uint32_t)ia | ((uint32_t)ib << 16);
                 tail = __builtin_amdgcn_readfirstlane(tail + (int)__popcll(bal));
                 if (tail - head >= 64) {
                     wave_sync();
@@ -462,15 +464,13 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                     const float Z = zx * zx + zy * zy + zz * zz;
                     // lanes past the list hold distinct far-away fillers: no pair with them is in range
                     const bool valid = k < 32 || lane < 32;
-                    bool in = false;
-                    if (valid) {
-                        if (fabsf(Z - a.rc2f) <= a.band3)
-                            in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xa.w),
-                                            __float_as_uint(wbuf[wvu][yi].w)) > a.rc2);
-                        else
-                            in = Z <= a.rc2f;
-                        in = accept(in, lane, yi);
-                    }
+                    bool in = valid && Z <= a.rc2f;
+                    // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
+                    const bool amb = valid && fabsf(Z - a.rc2f) <= a.band3;
+                    if (__ballot(amb) != 0ull && amb)
+                        in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xa.w),
+                                        __float_as_uint(wbuf[wvu][yi].w)) > a.rc2);
+                    in = accept(in, lane, yi);
                     push(in, lane, yi);
                 }
             } else {
