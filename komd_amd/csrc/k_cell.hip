@@ -65,10 +65,14 @@ struct CellArgs {
     double* __restrict__ vpart;    // [kSym][nblocks] virial tensor partials (energy mode)
     uint32_t* __restrict__ flags;
     unsigned long long* __restrict__ counters;
+    double* __restrict__ vx;       // second half-kick fused into the force write (kick > 0)
+    double* __restrict__ vy;
+    double* __restrict__ vz;
+    const double* __restrict__ mass;
     Grid g;
     int chunk, nblocks;
     float rc2f, band, sig2, eps24, eps4;
-    double rc2, limit;
+    double rc2, limit, kick;
 };
 
 __device__ __forceinline__ double image_shift(uint32_t code, double L) {
@@ -281,6 +285,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 a.fx[oi] = x;
                 a.fy[oi] = y;
                 a.fz[oi] = z;
+                if (a.kick > 0.0) kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, x, y, z);
             }
         }
     }
@@ -327,10 +332,14 @@ struct AtmArgs {
     double* __restrict__ part;
     double* __restrict__ vpart;        // [kSym][nblocks] virial tensor partials (energy mode)
     unsigned long long* __restrict__ counters;
+    double* __restrict__ vx;           // second half-kick fused into the force write (kick > 0)
+    double* __restrict__ vy;
+    double* __restrict__ vz;
+    const double* __restrict__ mass;
     Grid g;
     int n, chunk, nblocks, hyb_axis;
     float rc2f, band3, nu;
-    double rc2;
+    double rc2, kick;
 };
 
 constexpr int kAtmWaves = 4;
@@ -515,9 +524,11 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                     const double t = -de * (double)a.dlam[oi];
                     if (a.hyb_axis == 0) dx += t; else if (a.hyb_axis == 1) dy += t; else dz += t;
                 }
-                a.fx[oi] += dx;
-                a.fy[oi] += dy;
-                a.fz[oi] += dz;
+                const double nfx = a.fx[oi] + dx, nfy = a.fy[oi] + dy, nfz = a.fz[oi] + dz;
+                a.fx[oi] = nfx;
+                a.fy[oi] = nfy;
+                a.fz[oi] = nfz;
+                if (a.kick > 0.0) kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
                 if (a.esite) a.esite[oi] = HYB ? de * (double)li : de;
             }
             e_acc = HYB ? de * (double)li : de;
@@ -591,6 +602,8 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
     a.eps24 = 24.f * c->eps_u;
     a.eps4 = 4.f * c->eps_u;
     a.limit = c->prm.limit_factor;
+    a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
+    a.kick = (lj && !energy && c->kick_in == KICK_IN_CELL) ? 0.5 * c->prm.delta_t : 0.0;
     if (nl && (!d.nl || !d.nlc || (nl == 2 && !d.nld))) return hipErrorInvalidValue;
     const bool lim = limit && c->prm.limit_factor > 0;
     ScopedTimer tm(c, nl ? (lj ? "cell_lj_nl" : "cell_nl") : "cell_lj");
@@ -616,6 +629,8 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     a.rc2f = (float)a.rc2;
     a.band3 = 1e-5f * a.rc2f;
     a.nu = (float)c->prm.nu;
+    a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
+    a.kick = (!energy && c->kick_in == KICK_IN_ATM) ? 0.5 * c->prm.delta_t : 0.0;
     ScopedTimer tm(c, "atm");
     const bool hyb = lam != nullptr;
     if (hyb) {

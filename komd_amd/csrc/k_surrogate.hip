@@ -574,9 +574,14 @@ struct ForceArgs {
     float* __restrict__ fout;         // FORCE_OUT: forces [n][3], site order
     double* __restrict__ part;        // training: per-block loss partials
     double* __restrict__ vpart;       // energy: [kSym][vparts] virial partials
+    double* __restrict__ vx;          // second half-kick fused into the force write (kick > 0)
+    double* __restrict__ vy;
+    double* __restrict__ vz;
+    const double* __restrict__ mass;
     Grid g;
     int n, hyb_axis, chunk, vparts;
     float pirc, cF, lF;
+    double kick;
 };
 
 __device__ __forceinline__ int wrap_cell(const Grid& g, int x, int y, int z) {
@@ -760,9 +765,11 @@ __global__ __launch_bounds__(256) void k_sur_apply(ForceArgs a) {
                 const double t = (double)a.E[p] * (double)a.dlam[oi];
                 if (a.hyb_axis == 0) dx += t; else if (a.hyb_axis == 1) dy += t; else dz += t;
             }
-            a.fx[oi] += dx;
-            a.fy[oi] += dy;
-            a.fz[oi] += dz;
+            const double nfx = a.fx[oi] + dx, nfy = a.fy[oi] + dy, nfz = a.fz[oi] + dz;
+            a.fx[oi] = nfx;
+            a.fy[oi] = nfy;
+            a.fz[oi] = nfz;
+            if (a.kick > 0.0) kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
         } else if (MODE == FORCE_OUT) {
             a.fout[3 * (size_t)oi] = F.x;
             a.fout[3 * (size_t)oi + 1] = F.y;
@@ -1329,6 +1336,8 @@ hipError_t surrogate_pass(komd_ctx* c, bool energy, bool hybrid, int mode) {
         a.fout = s->fj;
         k_sur_apply<FORCE_OUT><<<site_blocks(c), 256, 0, c->stream>>>(a);
     } else {
+        a.vx = c->d.vx; a.vy = c->d.vy; a.vz = c->d.vz; a.mass = c->d.mass;
+        a.kick = (!energy && c->kick_in == KICK_IN_APPLY) ? 0.5 * c->prm.delta_t : 0.0;
         k_sur_apply<FORCE_APPLY><<<site_blocks(c), 256, 0, c->stream>>>(a);
     }
     return hipGetLastError();

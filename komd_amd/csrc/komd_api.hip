@@ -220,9 +220,19 @@ hipError_t enqueue_step(komd_ctx* c, int kind3b, bool bonded, bool thermo) {
     else
         KOMD_CHECK_HIP(launch_com_wrap_key(c, true));
     KOMD_CHECK_HIP(launch_bin(c));
-    KOMD_CHECK_HIP(enqueue_forces(c, kind3b, bonded));
-    KOMD_CHECK_HIP(launch_kick(c, thermo));
-    if (thermo) KOMD_CHECK_HIP(launch_beta_only(c));  // beta for the next step's rescale
+    // without the thermostat no kinetic sums are needed: the last force kernel of the
+    // step applies the second half-kick to the sites whose force it completes
+    c->kick_in = thermo ? KICK_NONE
+                        : (is_atm(kind3b) ? KICK_IN_ATM
+                                          : (kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID ? KICK_IN_APPLY
+                                                                                                      : KICK_IN_CELL));
+    const hipError_t e = enqueue_forces(c, kind3b, bonded);
+    c->kick_in = KICK_NONE;
+    KOMD_CHECK_HIP(e);
+    if (thermo) {
+        KOMD_CHECK_HIP(launch_kick(c, true));
+        KOMD_CHECK_HIP(launch_beta_only(c));  // beta for the next step's rescale
+    }
     return hipSuccess;
 }
 

@@ -121,6 +121,9 @@ struct komd_ctx {
     komd::Surrogate* sur = nullptr;
     // hybrid
     double run_t0 = 0;         // thermostat target of the running komd_run
+    // Integrator::Step1 fused into the last force kernel of an NVE step (no kinetic sums
+    // are needed there): 0 none (k_kick runs), else the kernel that applies it
+    int kick_in = 0;
     int hyb_axis = 0;
     double hyb_center = 0, hyb_half = 0, hyb_blend = 0;
     // exact-ATM judge of surrogate runs
@@ -155,6 +158,15 @@ hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift);
 hipError_t launch_bin(komd_ctx* c);             // scan + scatter + stable sort + FP32 repack
 hipError_t launch_kick_drift(komd_ctx* c);      // integrate0
 hipError_t launch_kick(komd_ctx* c, bool with_temp);  // integrate1 (+ kinetic partials)
+// v += f (dt/2)/m in the reference's FP64 operation order (Integrator.cpp:34-37)
+__device__ __forceinline__ void kick_site(double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                                          const double* __restrict__ mass, uint32_t i, double dt_half, double fx,
+                                          double fy, double fz) {
+    const double k = dt_half / mass[i];
+    vx[i] = __dadd_rn(vx[i], __dmul_rn(fx, k));
+    vy[i] = __dadd_rn(vy[i], __dmul_rn(fy, k));
+    vz[i] = __dadd_rn(vz[i], __dmul_rn(fz, k));
+}
 hipError_t launch_temperature(komd_ctx* c);     // kinetic partials -> scal[0..1]
 hipError_t launch_rescale(komd_ctx* c, double t0);    // beta from scal, v *= beta
 hipError_t launch_beta_only(komd_ctx* c);             // beta = sqrt(run_t0/T) into scal[2]
@@ -167,6 +179,7 @@ hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins);
 // the full lists, 2 lists + cell directions.  launch_atm: per-site ATM from the lists;
 // atm: 1 physical, 2 reference; lam/dlam: hybrid weights (nullptr otherwise); esite:
 // per-site energy share out (optional).
+enum { KICK_NONE = 0, KICK_IN_CELL = 1, KICK_IN_ATM = 2, KICK_IN_APPLY = 3 };
 hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit);
 hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite);
 int cell_blocks(komd_ctx* c);
