@@ -105,37 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_com_wrap_key(
     }
 }
 
-// Exclusive scan of the per-cell counts by one workgroup; zeroes the counts for
-// the next histogram (no memset node is needed when a step is graph-captured).
-constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_scan(int ncell, uint32_t* __restrict__ count,
-                                                       uint32_t* __restrict__ start) {
-    __shared__ uint32_t part[kScanThreads];
-    const int t = threadIdx.x;
-    const int chunk = (ncell + kScanThreads - 1) / kScanThreads;
-    const int b = t * chunk;
-    const int e = min(ncell, b + chunk);
-    uint32_t s = 0;
-    for (int i = b; i < e; i++) s += count[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t v = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
-    for (int i = b; i < e; i++) {
-        const uint32_t c = count[i];
-        start[i] = run;
-        run += c;
-        count[i] = 0;
-    }
-    if (t == kScanThreads - 1) start[ncell] = part[t];
-}
-
-// Large grids: tiles of 1024 cells, coalesced.  k_scan_tiles sums each tile;
+// Grids above kFuseCells cells: tiles of 1024 cells, coalesced.  k_scan_tiles sums each tile;
 // k_scan_fin offsets a tile by the sum of the tiles before it, scans it in LDS,
 // writes start[] and zeroes the counts.
 constexpr int kScanTile = 1024;
@@ -191,6 +161,50 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __res
     tmp[start[key[i]] + slot[i]] = (uint32_t)i;
 }
 
+// Grids of at most kFuseCells cells: every workgroup scans the counts itself in LDS
+// (block 0 also writes start[]) and scatters its sites, so no separate scan launch
+// is needed; k_sort_repack zeroes the counts afterwards.
+constexpr int kFuseCells = 4096;
+__global__ __launch_bounds__(kBlock) void k_scan_scatter(int n, int ncell, const uint32_t* __restrict__ count,
+                                                         const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ slot,
+                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
+    __shared__ uint32_t pre[kFuseCells + 1];
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int chunk = (ncell + kBlock - 1) / kBlock;
+    const int b0 = min(ncell, t * chunk), b1 = min(ncell, b0 + chunk);
+    uint32_t s = 0;
+    for (int i = b0; i < b1; i++) {
+        const uint32_t c = count[i];
+        pre[i] = c;
+        s += c;
+    }
+    uint32_t v = s;  // inclusive scan of the chunk sums across the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    uint32_t run = v - s, total = 0;
+    for (int k = 0; k < kBlock / 64; k++) {
+        if (k < w) run += wsum[k];
+        total += wsum[k];
+    }
+    for (int i = b0; i < b1; i++) {
+        const uint32_t c = pre[i];
+        pre[i] = run;
+        run += c;
+    }
+    if (t == 0) pre[ncell] = total;
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int i = t; i <= ncell; i += kBlock) start[i] = pre[i];
+    const int i = blockIdx.x * blockDim.x + t;
+    if (i < n) tmp[pre[key[i]] + slot[i]] = (uint32_t)i;
+}
+
 // Stable order inside a cell (ascending site index, as Cell::addIndex appends in
 // site order, Cell.cpp:32-34) and the FP32 repack of the cell-sorted positions
 // relative to the cell centre.
@@ -199,8 +213,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_repack(
     uint32_t* __restrict__ sorted, const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ rz, float4* __restrict__ spos, const double* __restrict__ eps,
     const double* __restrict__ sig, float2* __restrict__ sprm, const uint32_t* __restrict__ idcls,
-    uint32_t* __restrict__ sid, uint32_t* __restrict__ inv, Grid g) {
+    uint32_t* __restrict__ sid, uint32_t* __restrict__ inv, Grid g, uint32_t* __restrict__ zero_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero_count && i < g.ncell) zero_count[i] = 0;  // histogram of the next binning
     if (i >= n) return;
     const uint32_t k = key[i];
     const uint32_t s = start[k], e = start[k + 1];
@@ -382,29 +397,32 @@ hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev) {
 hipError_t launch_bin(komd_ctx* c) {
     auto& d = c->d;
     const int n = (int)c->n;
-    {
-        ScopedTimer tm(c, "bin_scan");
-        const int C = c->grid.ncell;
-        if (C <= 4 * kScanTile) {
-            k_scan<<<1, kScanThreads, 0, c->stream>>>(C, d.count, d.start);
-        } else {
+    const int C = c->grid.ncell;
+    const bool fused = C <= kFuseCells;
+    if (fused) {
+        ScopedTimer tm(c, "bin_scatter");
+        k_scan_scatter<<<std::max(1, (n + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(n, C, d.count, d.key, d.slot,
+                                                                                       d.start, d.tmp);
+        KOMD_CHECK_HIP(hipGetLastError());
+    } else {
+        {
+            ScopedTimer tm(c, "bin_scan");
             const int tiles = (C + kScanTile - 1) / kScanTile;
             k_scan_tiles<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.scanp);
             KOMD_CHECK_HIP(hipGetLastError());
             k_scan_fin<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.start, d.scanp);
+            KOMD_CHECK_HIP(hipGetLastError());
         }
-        KOMD_CHECK_HIP(hipGetLastError());
-    }
-    {
         ScopedTimer tm(c, "bin_scatter");
         k_scatter<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(n, d.key, d.slot, d.start, d.tmp);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     {
         ScopedTimer tm(c, "bin_sort");
-        k_sort_repack<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+        const int units = std::max(n, fused ? C : 0);
+        k_sort_repack<<<(units + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
             n, d.key, d.start, d.tmp, d.sorted, d.rx, d.ry, d.rz, d.spos, d.eps, d.sig, c->uniform ? nullptr : d.sprm,
-            d.idcls, c->multisite ? d.sid : nullptr, d.inv, c->grid);
+            d.idcls, c->multisite ? d.sid : nullptr, d.inv, c->grid, fused ? d.count : nullptr);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     return hipSuccess;
