@@ -207,23 +207,18 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
             // every lane of the workgroup runs the same trip count (ballots need the whole wave)
             for (int t0 = 0; t0 < mc; t0 += lps) {
                 const int t = t0 + sub;
-                bool hit = false;
-                float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 0.f;
-                uint32_t w = 0;
-                if (active && t < mc) {
-                    const float4 c = pos[t];
-                    w = __float_as_uint(c.w);
-                    const uint32_t oj = w & kIdxMask;
-                    dx = c.x - qi.x; dy = c.y - qi.y; dz = c.z - qi.z;
-                    r2 = dx * dx + dy * dy + dz * dz;
-                    bool in;
-                    if (fabsf(r2 - a.rc2f) <= a.band)
-                        in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
-                    else
-                        in = r2 <= a.rc2f;
-                    hit = in && (oj != oi);
-                    if (hit && oj > oi) npairs++;
-                }
+                const bool cand = active && t < mc;
+                const float4 c = pos[cand ? t : 0];
+                const uint32_t w = __float_as_uint(c.w);
+                const uint32_t oj = w & kIdxMask;
+                const float dx = c.x - qi.x, dy = c.y - qi.y, dz = c.z - qi.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                bool in = cand && r2 <= a.rc2f;
+                // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
+                const bool amb = cand && fabsf(r2 - a.rc2f) <= a.band;
+                if (__ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
+                const bool hit = in && (oj != oi);
+                npairs += (unsigned long long)__popcll(__ballot(hit && oj > oi));  // wave-uniform
                 if (NL) {
                     const unsigned long long bal = (__ballot(hit) >> gshift) & gmask;
                     if (hit) {
@@ -290,9 +285,9 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
         }
     }
 
-    // in-range pair count: one slot per workgroup, accumulated across launches
+    // in-range pair count (the same in every lane of a wave): one slot per workgroup,
+    // accumulated across launches
     // (a single global counter would serialise thousands of same-address atomics)
-    npairs = (unsigned long long)wave_sum((double)npairs);
     if (lane == 0) red_cnt[wv] = npairs;
     __syncthreads();
     if (tid == 0) a.counters[b] += red_cnt[0] + red_cnt[1] + red_cnt[2] + red_cnt[3];
