@@ -45,6 +45,14 @@ NU = 0.073
 FLOP_PAIR = 26      # per unique in-range LJ pair
 FLOP_TRIPLET = 147  # per unique in-range ATM triplet
 FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK = 8.0e12  # B/s
+# algorithmic HBM bytes per site of the binning / integration kernels (FP64 state, u32 indices):
+#   bin_key     Integrator::Step0 + container update: read r v f 72 + mass 8, write r v 48,
+#               f = 0 24, cell key and arrival slot 8
+#   bin_scatter read key and slot 8, write the CSR slot 4
+#   bin_sort    read key 4 + its CSR entry 4 + r 24, write sorted, inverse map 8 and FP32 position 16
+#   kick        (thermostatted steps only) Integrator::Step1: read v f mass 56, write v 24
+KERNEL_BYTES_PER_SITE = {"bin_key": 160, "bin_scatter": 12, "bin_sort": 56, "kick": 80}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "atm_32000_traffic.json")
 WEIGHTS = os.path.join(ROOT, "komd_amd", "weights", "atm_surrogate.npy")
 RHO = 0.8
@@ -78,6 +86,12 @@ def rel_err(a, b):
     scale = float(np.sqrt(np.mean(b * b)))
     d = np.linalg.norm(a - b, axis=1)
     return {"max": float(d.max() / scale), "rms": float(np.sqrt(np.mean(d * d)) / scale)}
+
+
+def hbm_rates(kern_us, n):
+    """Achieved GB/s of the binning / integration kernels from their algorithmic bytes."""
+    return {k: round(KERNEL_BYTES_PER_SITE[k] * n / (us * 1e-6) / 1e9, 1)
+            for k, us in kern_us.items() if k in KERNEL_BYTES_PER_SITE and us > 0}
 
 
 def drift_of(ctx, K, n, kind, steps):
@@ -149,13 +163,14 @@ def learned_leg(n_atoms, args):
             ctx.timing(True)
             ctx.run(min(args.steps, 200), kind3b=kind)
             kern = {}
-            for k in ("cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
+            for k in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj_nl", "atm", "sur_desc", "sur_mlp",
+                      "sur_force", "sur_apply"):
                 ms, cnt = ctx.timer(k)
                 if cnt:
                     kern[k] = round(ms / cnt * 1e3, 2)
             ctx.timing(False)
             legs[name] = {"value": round(n * args.steps / dt, 1), "ms_per_step": round(dt / args.steps * 1e3, 4),
-                          "kernels_us": kern}
+                          "kernels_us": kern, "binning_GBps": hbm_rates(kern, n)}
         out["exact"], out["surrogate"] = legs["exact"], legs["surrogate"]
         out["speedup_surrogate_vs_exact"] = round(legs["surrogate"]["value"] / legs["exact"]["value"], 3)
         out["three_body_part_us"] = {
@@ -393,7 +408,14 @@ def main():
     kernels = {}
     for name in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj_nl", "atm", "kick"):
         ms, cnt = ctx.timer(name)
+        if not cnt:  # fused: the cell-count scan into the scatter (grids up to 4096 cells), the
+            continue  # NVE second half-kick into the three-body kernel
         kernels[name] = {"us_per_launch": round(ms / max(cnt, 1) * 1e3, 2), "launches": cnt}
+        if name in KERNEL_BYTES_PER_SITE:
+            b = KERNEL_BYTES_PER_SITE[name] * n
+            kernels[name].update({"algorithmic_bytes": b,
+                                  "achieved_GBps": round(b / (ms / cnt * 1e-3) / 1e9, 1),
+                                  "frac_of_hbm_peak": round(b / (ms / cnt * 1e-3) / HBM_PEAK, 4)})
     atm_ms, atm_n = ctx.timer("atm")
     cell_ms, cell_n = ctx.timer("cell_lj_nl")
     ctx.timing(False)
