@@ -79,20 +79,23 @@ __device__ __forceinline__ double image_shift(uint32_t code, double L) {
     return code == 1u ? L : (code == 2u ? -L : 0.0);
 }
 
-// |(r_a + s_a) - (r_b + s_b)|^2 in FP64, reference operation order, no contraction.
+// |(r_a + s_a) - (r_b + s_b)|^2 in FP64, reference operation order, no contraction
+// (the pragma covers the operators written here; __dmul_rn / __dadd_rn would not be
+// covered, their bodies are plain * and + compiled with contraction on).
 __device__ __forceinline__ double exact_r2(const double* __restrict__ rx, const double* __restrict__ ry,
                                            const double* __restrict__ rz, const Grid& g, uint32_t wa, uint32_t wb) {
+#pragma clang fp contract(off)
     const uint32_t i = wa & kIdxMask, j = wb & kIdxMask, ca = wa >> kIdxBits, cb = wb >> kIdxBits;
-    const double xa = __dadd_rn(rx[i], image_shift(ca & 3u, g.size[0]));
-    const double ya = __dadd_rn(ry[i], image_shift((ca >> 2) & 3u, g.size[1]));
-    const double za = __dadd_rn(rz[i], image_shift((ca >> 4) & 3u, g.size[2]));
-    const double xb = __dadd_rn(rx[j], image_shift(cb & 3u, g.size[0]));
-    const double yb = __dadd_rn(ry[j], image_shift((cb >> 2) & 3u, g.size[1]));
-    const double zb = __dadd_rn(rz[j], image_shift((cb >> 4) & 3u, g.size[2]));
-    const double dx = __dsub_rn(xa, xb), dy = __dsub_rn(ya, yb), dz = __dsub_rn(za, zb);
-    double s = __dmul_rn(dx, dx);
-    s = __dadd_rn(s, __dmul_rn(dy, dy));
-    s = __dadd_rn(s, __dmul_rn(dz, dz));
+    const double xa = rx[i] + image_shift(ca & 3u, g.size[0]);
+    const double ya = ry[i] + image_shift((ca >> 2) & 3u, g.size[1]);
+    const double za = rz[i] + image_shift((ca >> 4) & 3u, g.size[2]);
+    const double xb = rx[j] + image_shift(cb & 3u, g.size[0]);
+    const double yb = ry[j] + image_shift((cb >> 2) & 3u, g.size[1]);
+    const double zb = rz[j] + image_shift((cb >> 4) & 3u, g.size[2]);
+    const double dx = xa - xb, dy = ya - yb, dz = za - zb;
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
     return s;
 }
 

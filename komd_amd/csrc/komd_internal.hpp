@@ -158,14 +158,18 @@ hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift);
 hipError_t launch_bin(komd_ctx* c);             // scan + scatter + stable sort + FP32 repack
 hipError_t launch_kick_drift(komd_ctx* c);      // integrate0
 hipError_t launch_kick(komd_ctx* c, bool with_temp);  // integrate1 (+ kinetic partials)
-// v += f (dt/2)/m in the reference's FP64 operation order (Integrator.cpp:34-37)
+// v += f (dt/2)/m in the reference's FP64 operation order (Integrator.cpp:34-37).
+// Contraction into an FMA is switched off for the operators written in this body
+// (the pragma does not reach into other functions, e.g. __dmul_rn / __dadd_rn, whose
+// bodies are plain * and + compiled with contraction on).
 __device__ __forceinline__ void kick_site(double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
                                           const double* __restrict__ mass, uint32_t i, double dt_half, double fx,
                                           double fy, double fz) {
+#pragma clang fp contract(off)
     const double k = dt_half / mass[i];
-    vx[i] = __dadd_rn(vx[i], __dmul_rn(fx, k));
-    vy[i] = __dadd_rn(vy[i], __dmul_rn(fy, k));
-    vz[i] = __dadd_rn(vz[i], __dmul_rn(fz, k));
+    vx[i] = vx[i] + fx * k;
+    vy[i] = vy[i] + fy * k;
+    vz[i] = vz[i] + fz * k;
 }
 hipError_t launch_temperature(komd_ctx* c);     // kinetic partials -> scal[0..1]
 hipError_t launch_rescale(komd_ctx* c, double t0);    // beta from scal, v *= beta

@@ -241,6 +241,33 @@ def test_graph_replays_match_eager(orc):
             assert np.abs(ra - rb).max() < 1e-8 and np.abs(va - vb).max() < 1e-6
 
 
+@pytest.mark.parametrize("kind", [0, K.ATM_EXACT])
+def test_fused_kick_matches_functor_loop(orc, kind):
+    """NVE komd_run applies Integrator::Step1 inside its last force kernel (k_cell for
+    the pair-only step, k_atm with three-body forces).  The reference's per-functor
+    loop (integrate0, updateContainer, LJ12_6, Limit, ATM, integrate1;
+    Simulation.cpp:49-74) runs the same kernels with a separate k_kick; every sum has a
+    fixed order, so both must give the same bits after 12 steps."""
+    s = small(91, 6, jitter=0.05)
+    with K.Context(s, nu=NU, limit_factor=20.0) as a, K.Context(s, nu=NU, limit_factor=20.0) as b:
+        a.compute_forces(kind)
+        b.compute_forces(kind)
+        a.run(12, kind3b=kind)                      # graph path, fused half-kick
+        for _ in range(12):
+            b.integrate0()
+            b.update_container()
+            b.forces_pair()
+            b.forces_limit()
+            if kind:
+                b.forces_3b(kind)
+            b.integrate1()
+        ra, va, fa = a.get_state()
+        rb, vb, fb = b.get_state()
+        assert np.array_equal(fa, fb)
+        assert np.array_equal(va, vb)
+        assert np.array_equal(ra, rb)
+
+
 def test_site_binning_and_chain_lj(orc):
     """KOMD_BIN_SITE: every site binned and wrapped on its own position (the oracle's
     binning with one molecule per site), and LJ between different molecules equal to the
