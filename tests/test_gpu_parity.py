@@ -140,6 +140,28 @@ def test_lj_mixing_and_exclusion(orc):
     assert mx < 2e-4 and rms < 5e-6
 
 
+def test_cutoff_decision_unfused(orc):
+    """27 isolated site pairs within an ulp of rc, each decided differently by a fused
+    and an unfused FP64 distance: the GPU keeps exactly the pairs the reference order
+    (and the FP64 oracle) keeps."""
+    from borderline import borderline_system
+    s, want = borderline_system()
+    rb, b = orc.update_container(s)
+    f_or = orc.lj_c08(s, rb, b)
+    with K.Context(s) as c:
+        c.update_container()
+        c.counters(reset=True)
+        c.forces_pair()
+        f = c.forces()
+        npair, _ = c.counters(reset=True)
+    got = [bool(np.abs(f[2 * k]).max() > 0) for k in range(len(want))]
+    ora = [bool(np.abs(f_or[2 * k]).max() > 0) for k in range(len(want))]
+    assert 0 < sum(want) < len(want)
+    assert ora == want
+    assert got == want
+    assert npair == sum(want)
+
+
 @pytest.mark.parametrize("cells,seed", [(5, 41), (10, 42), (20, 43)])
 def test_atm_exact_forces(orc, cells, seed):
     s = small(seed, cells)

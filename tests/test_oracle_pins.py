@@ -137,6 +137,20 @@ def test_lj_c08_equals_brute_force():
         assert np.abs(f.sum(0)).max() < 1e-9
 
 
+def test_lj_cutoff_decision_is_unfused():
+    """Mistake caught: an FMA-contracted distance in the oracle (a build without
+    -ffp-contract=off).  For pairs within an ulp of rc, the kept set must be the one
+    of the written FP64 order (dx dx + dy dy) + dz dz <= rc^2 (LJ12_6.cpp:24-28),
+    which for each of these pairs differs from the fused sum's."""
+    from borderline import borderline_system
+    s, want = borderline_system()
+    r, b = O.update_container(s)
+    f = O.lj_c08(s, r, b)
+    kept = [bool(np.abs(f[2 * k]).max() > 0) for k in range(len(want))]
+    assert 0 < sum(want) < len(want)
+    assert kept == want
+
+
 def test_lj_same_molecule_excluded():
     """Two 2-site molecules in neighbouring cells: only the 4 inter-molecular pairs act
     (LJ12_6.cpp:19); catches a missing or inverted id test."""
