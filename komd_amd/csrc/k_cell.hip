@@ -139,26 +139,32 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     if (live) {
         const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
         const int hx = cell % nx, hy = (cell / nx) % ny, hz = cell / (nx * ny);
-        if (tid < 27) {
-            const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
-            int x = hx + dx, y = hy + dy, z = hz + dz;
-            uint32_t cx = 0, cy = 0, cz = 0;
-            double sx = 0, sy = 0, sz = 0;
-            if (x < 0) { x += nx; cx = 2; sx = -a.g.size[0]; } else if (x >= nx) { x -= nx; cx = 1; sx = a.g.size[0]; }
-            if (y < 0) { y += ny; cy = 2; sy = -a.g.size[1]; } else if (y >= ny) { y -= ny; cy = 1; sy = a.g.size[1]; }
-            if (z < 0) { z += nz; cz = 2; sz = -a.g.size[2]; } else if (z >= nz) { z -= nz; cz = 1; sz = a.g.size[2]; }
-            const int nc = x + nx * (y + ny * z);
-            nb_start[tid] = (int)a.start[nc];
-            nb_pref[tid + 1] = (int)(a.start[nc + 1] - a.start[nc]);
-            nb_off[tid][0] = (float)((double)(x - hx) * a.g.cs + sx);
-            nb_off[tid][1] = (float)((double)(y - hy) * a.g.cs + sy);
-            nb_off[tid][2] = (float)((double)(z - hz) * a.g.cs + sz);
-            nb_code[tid] = (cx | (cy << 2) | (cz << 4)) << kIdxBits;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            nb_pref[0] = 0;
-            for (int k = 0; k < 27; k++) nb_pref[k + 1] += nb_pref[k];
+        if (tid < 64) {  // wave 0: the 27 neighbour cells and the prefix of their counts
+            int cnt = 0;
+            if (tid < 27) {
+                const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
+                int x = hx + dx, y = hy + dy, z = hz + dz;
+                uint32_t cx = 0, cy = 0, cz = 0;
+                double sx = 0, sy = 0, sz = 0;
+                if (x < 0) { x += nx; cx = 2; sx = -a.g.size[0]; } else if (x >= nx) { x -= nx; cx = 1; sx = a.g.size[0]; }
+                if (y < 0) { y += ny; cy = 2; sy = -a.g.size[1]; } else if (y >= ny) { y -= ny; cy = 1; sy = a.g.size[1]; }
+                if (z < 0) { z += nz; cz = 2; sz = -a.g.size[2]; } else if (z >= nz) { z -= nz; cz = 1; sz = a.g.size[2]; }
+                const int nc = x + nx * (y + ny * z);
+                nb_start[tid] = (int)a.start[nc];
+                cnt = (int)(a.start[nc + 1] - a.start[nc]);
+                nb_off[tid][0] = (float)((double)(x - hx) * a.g.cs + sx);
+                nb_off[tid][1] = (float)((double)(y - hy) * a.g.cs + sy);
+                nb_off[tid][2] = (float)((double)(z - hz) * a.g.cs + sz);
+                nb_code[tid] = (cx | (cy << 2) | (cz << 4)) << kIdxBits;
+            }
+            int incl = cnt;  // inclusive scan across the wave
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                const int o = __shfl_up(incl, off);
+                if (tid >= off) incl += o;
+            }
+            if (tid < 27) nb_pref[tid + 1] = incl;
+            if (tid == 0) nb_pref[0] = 0;
         }
         __syncthreads();
         // workgroup-uniform values read from LDS: keep them in scalar registers

@@ -612,22 +612,27 @@ __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
         return;
     }
     const int hx = cell % a.g.nx, hy = (cell / a.g.nx) % a.g.ny, hz = cell / (a.g.nx * a.g.ny);
-    if (tid < 27) {
-        const int nc = wrap_cell(a.g, hx + tid % 3 - 1, hy + (tid / 3) % 3 - 1, hz + tid / 9 - 1);
-        nb_start[tid] = (int)a.start[nc];
-        nb_cnt[tid] = (int)(a.start[nc + 1] - a.start[nc]);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int ok = 1;
-        nb_base[0] = 0;
-        for (int k = 0; k < 27; k++) {
-            nb_base[k + 1] = nb_base[k] + nb_cnt[k];
-            if (nb_cnt[k] > kRows) ok = 0;
+    if (tid < 64) {  // wave 0: the 27 neighbour cells, the prefix of their counts, capacity check
+        int cnt = 0;
+        if (tid < 27) {
+            const int nc = wrap_cell(a.g, hx + tid % 3 - 1, hy + (tid / 3) % 3 - 1, hz + tid / 9 - 1);
+            cnt = (int)(a.start[nc + 1] - a.start[nc]);
+            nb_start[tid] = (int)a.start[nc];
+            nb_cnt[tid] = cnt;
         }
-        if (nb_base[27] > kSurM) ok = 0;
-        bad = !ok;
-        if (!ok) atomicOr(a.flags, FLAG_CELL_OVERFLOW);
+        int incl = cnt;  // inclusive scan across the wave
+#pragma unroll
+        for (int off = 1; off < 32; off <<= 1) {
+            const int o = __shfl_up(incl, off);
+            if (tid >= off) incl += o;
+        }
+        if (tid < 27) nb_base[tid + 1] = incl;
+        const bool over = __ballot(tid < 27 && cnt > kRows) != 0ull || __shfl(incl, 26) > kSurM;
+        if (tid == 0) {
+            nb_base[0] = 0;
+            bad = over;
+            if (over) atomicOr(a.flags, FLAG_CELL_OVERFLOW);
+        }
     }
     __syncthreads();
     if (bad) {
