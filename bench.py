@@ -19,8 +19,12 @@ Prints one JSON line:
                pairs, ATM_NOLIST triplet walk, per-interaction atomics), timed on
                this host on a bounded sample of the same workload
   force_error  max / RMS |F_gpu - F_fp64| / RMS(F_fp64) on an equilibrated
-               snapshot, total force and three-body force separately
+               snapshot, total force and three-body force separately (the FP64
+               forces come from the same CPU leg as cpu_baseline)
   energy_drift NVE drift of the total energy per atom over --drift-steps steps
+  energy_drift_vs_fp64
+               the GPU path's drift beside the FP64 run of the same 4,000-atom
+               LJ + ATM system over 10,000 steps (stored fixture)
   learned      the learned three-body path (descriptor-MLP surrogate, weights in
                komd_amd/weights/) at 32,000 and 256,000 atoms: throughput beside the
                exact path on the same state, its three-body force error against the
@@ -54,6 +58,7 @@ HBM_PEAK = 8.0e12  # B/s
 #   kick        (thermostatted steps only) Integrator::Step1: read v f mass 56, write v 24
 KERNEL_BYTES_PER_SITE = {"bin_key": 160, "bin_scatter": 12, "bin_sort": 56, "kick": 80}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "atm_32000_traffic.json")
+DRIFT_FIXTURE = os.path.join(ROOT, "tests", "golden", "drift_fp64_fcc4000_atm.json")
 WEIGHTS = os.path.join(ROOT, "komd_amd", "weights", "atm_surrogate.npy")
 RHO = 0.8
 
@@ -337,6 +342,73 @@ def other_legs(args):
     return out
 
 
+def cpu_reference_leg(sysm, snapshot, args):
+    """The CPU side, the only place the bench runs the FP64 oracle (checker and
+    baseline, never the measured path):
+      cpu_baseline  the oracle's reference-order loop (C08 cell pairs, ATM_NOLIST triplet
+                    walk, FP64 atomics) timed on a bounded sample of the workload;
+      force_error   FP64 forces of the GPU snapshot's positions against the GPU's."""
+    cpu = force_error = None
+    if args.no_cpu and snapshot is None:
+        return cpu, force_error
+    import oracle
+    threads = os.cpu_count() or 1
+    n = sysm.n
+    if not args.no_cpu:
+        oracle.run(sysm, 1, dt=0.005, nu=NU, atm=1, atm_sign=-1.0, nthreads=threads)  # warm (first binning)
+        c0 = time.perf_counter()
+        oracle.run(sysm, args.cpu_steps, dt=0.005, nu=NU, atm=1, atm_sign=-1.0, nthreads=threads)
+        c1 = time.perf_counter()
+        cpu = {"value": round(n * args.cpu_steps / (c1 - c0), 1), "unit": "atom-timesteps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the {n}-atom LJ+ATM workload after 1 warm step: C08 cell "
+                         f"pairs (LinkedCells.cpp:164-224) + ATM_NOLIST triplet walk (ATM_NOLIST.cpp:20-159) with "
+                         f"FP64 atomics, {threads} OpenMP threads, {c1 - c0:.2f} s wall"}
+    if snapshot is not None:
+        r_snap, v_snap, f_tot, f_atm = snapshot
+        snap = sysm.copy()
+        snap.r, snap.v = r_snap, v_snap
+        r, b = oracle.update_container(snap)
+        f_lj = oracle.lj_c08(snap, r, b, nthreads=threads)
+        f_3b, _, _ = oracle.atm_complete(snap, r, b, NU, coverage=0, sign=-1.0, nthreads=threads)
+        force_error = {"total": rel_err(f_tot, f_lj + f_3b), "three_body": rel_err(f_atm, f_3b),
+                       "snapshot": f"after {args.equil_steps} NVE steps from the lattice",
+                       "reference": "FP64 oracle (C restatement) on the same positions"}
+    return cpu, force_error
+
+
+def drift_against_fp64(K):
+    """NVE energy trajectory of the GPU path against the FP64 run of the same system
+    (4,000 atoms FCC 10^3 x 4, rho 0.8, T* 0.8, LJ + complete ATM, 10,000 steps), whose
+    E/N every 100 steps is stored in tests/golden/drift_fp64_fcc4000_atm.json (written by
+    tests/golden/make_drift_fp64.py on the CPU)."""
+    from komd_amd import systems as S
+    if not os.path.exists(DRIFT_FIXTURE):
+        return None
+    with open(DRIFT_FIXTURE) as fh:
+        ref = json.load(fh)
+    s = S.fcc(10, 0.8, 0.8, 1003)
+    every, steps = ref["every"], ref["steps"]
+    e = []
+    with K.Context(s, cutoff=2.5, dt=0.005, nu=NU) as ctx:
+        ctx.compute_forces(K.ATM_EXACT)
+        for k, st in enumerate(steps):
+            if k:
+                ctx.run(st - steps[k - 1], kind3b=K.ATM_EXACT)
+            o = ctx.observables(K.ATM_EXACT)
+            e.append((o["kinetic"] + o["u_lj"] + o["u_3b"]) / s.n)
+    t = np.array(steps) * 0.005
+    e = np.array(e)
+    eref = np.array(ref["e_per_atom"])
+    return {"system": ref["system"], "steps": steps[-1],
+            "gpu": {"slope_e_per_atom_per_tau": float(np.polyfit(t, e, 1)[0]),
+                    "max_dev_per_atom": float(np.max(np.abs(e - e[0])))},
+            "fp64": {"slope_e_per_atom_per_tau": ref["slope_e_per_atom_per_tau"],
+                     "max_dev_per_atom": ref["max_dev_per_atom"]},
+            "e_per_atom_start": {"gpu": float(e[0]), "fp64": float(eref[0])},
+            "max_abs_diff_first_1000_steps": float(np.max(np.abs(e[:11] - eref[:11])))}
+
+
 def copy_calibration(mib: int):
     """A plain device copy of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE read on
     it give the factor that turns those counters into HBM bytes."""
@@ -369,26 +441,17 @@ def main():
     ctx = K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU)
     ctx.compute_forces(K.ATM_EXACT)
 
-    # ---- force error against FP64 on an equilibrated snapshot (oracle = checker only)
-    force_error = None
+    # ---- GPU forces on an equilibrated snapshot; the FP64 reference forces of the same
+    # positions are computed in the CPU reference leg at the end
+    snapshot = None
     if not args.no_check:
-        import oracle
         ctx.run(args.equil_steps, kind3b=K.ATM_EXACT)
         r_snap, v_snap, _ = ctx.get_state()
-        snap = sysm.copy()
-        snap.r, snap.v = r_snap, v_snap
-        threads = os.cpu_count() or 1
-        r, b = oracle.update_container(snap)
-        f_lj = oracle.lj_c08(snap, r, b, nthreads=threads)
-        f_3b, _, _ = oracle.atm_complete(snap, r, b, NU, coverage=0, sign=-1.0, nthreads=threads)
         ctx.compute_forces(K.ATM_EXACT)
         f_tot = ctx.forces()
         ctx.update_container()
         ctx.forces_3b(K.ATM_EXACT)
-        f_atm = ctx.forces()
-        force_error = {"total": rel_err(f_tot, f_lj + f_3b), "three_body": rel_err(f_atm, f_3b),
-                       "snapshot": f"after {args.equil_steps} NVE steps from the lattice",
-                       "reference": "FP64 oracle (C restatement) on the same positions"}
+        snapshot = (r_snap, v_snap, f_tot, ctx.forces())
         ctx.compute_forces(K.ATM_EXACT)
 
     # ---- throughput: K graph-replayed steps, inputs resident in HBM
@@ -459,20 +522,8 @@ def main():
         hybrid = hybrid_leg(args)
     other = None if args.no_other else other_legs(args)
 
-    # ---- CPU baseline: FP64 OpenMP restatement of the reference loop, bounded sample
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        threads = os.cpu_count() or 1
-        oracle.run(sysm, 1, dt=0.005, nu=NU, atm=1, atm_sign=-1.0, nthreads=threads)  # warm (first binning)
-        c0 = time.perf_counter()
-        oracle.run(sysm, args.cpu_steps, dt=0.005, nu=NU, atm=1, atm_sign=-1.0, nthreads=threads)
-        c1 = time.perf_counter()
-        cpu = {"value": round(n * args.cpu_steps / (c1 - c0), 1), "unit": "atom-timesteps/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the {n}-atom LJ+ATM workload after 1 warm step: C08 cell "
-                         f"pairs (LinkedCells.cpp:164-224) + ATM_NOLIST triplet walk (ATM_NOLIST.cpp:20-159) with "
-                         f"FP64 atomics, {threads} OpenMP threads, {c1 - c0:.2f} s wall"}
+    cpu, force_error = cpu_reference_leg(sysm, snapshot, args)
+    drift_fp64 = None if args.drift_steps <= 0 else drift_against_fp64(K)
 
     cells = "20^3" if n == 32000 else "40^3"
     out = {
@@ -487,6 +538,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "force_error": force_error,
+        "energy_drift_vs_fp64": drift_fp64,
         "energy_drift": drift,
         "kernels": kernels,
         "learned": learned or None,

@@ -1,0 +1,69 @@
+"""FP64 energy trajectory of the exact three-body argon system from the oracle (CPU).
+
+System: 4,000 argon atoms (FCC 10^3 x 4, rho 0.8, T* 0.8, seed 1003), LJ 12-6 rc 2.5 +
+Axilrod-Teller-Muto nu 0.073 over every triplet within rc (physical sign, -grad V),
+velocity Verlet dt 0.005, NVE, 10,000 steps, all in FP64 in the reference's operation
+order (oracle: vv_step0 -> update_container -> lj_c08 + atm_complete -> vv_step1).
+Writes E_total / N every 100 steps and the slope of a linear fit against time.
+
+  python tests/golden/make_drift_fp64.py [--steps 10000] [--threads 8]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle as O  # noqa: E402
+from komd_amd import systems as S  # noqa: E402
+
+NU, DT, EVERY = 0.073, 0.005, 100
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--out", default=os.path.join(HERE, "drift_fp64_fcc4000_atm.json"))
+    a = ap.parse_args()
+    O.build()
+    s = S.fcc(10, 0.8, 0.8, 1003)
+    n = s.n
+
+    def forces(r):
+        rb, b = O.update_container(s, r)
+        f, u, _ = O.lj_c08(s, rb, b, energy=True, nthreads=a.threads)
+        fa, ea, _ = O.atm_complete(s, rb, b, NU, coverage=0, sign=-1.0, nthreads=a.threads)
+        return rb, f + fa, u + ea
+
+    r, v = s.r.copy(), s.v.copy()
+    r, f, U = forces(r)
+    steps, e = [0], [(0.5 * float(np.sum(s.mass[:, None] * v * v)) + U) / n]
+    t0 = time.time()
+    for k in range(1, a.steps + 1):
+        r, v = O.vv_step0(DT, r, v, f, s.mass)
+        r, f, U = forces(r)
+        v = O.vv_step1(DT, v, f, s.mass)
+        if k % EVERY == 0:
+            steps.append(k)
+            e.append((0.5 * float(np.sum(s.mass[:, None] * v * v)) + U) / n)
+            if k % 1000 == 0:
+                print(f"step {k}  E/N {e[-1]:.10f}  {time.time() - t0:.0f}s", flush=True)
+    t = np.array(steps) * DT
+    slope = float(np.polyfit(t, np.array(e), 1)[0])
+    out = {"system": "4,000 argon atoms, FCC 10^3 x 4, rho 0.8, T* 0.8 (seed 1003), LJ rc 2.5 + ATM nu 0.073 "
+                     "(every triplet within rc, -grad V), dt 0.005, NVE",
+           "precision": "FP64 throughout (oracle)", "every": EVERY, "steps": steps, "e_per_atom": e,
+           "slope_e_per_atom_per_tau": slope, "max_dev_per_atom": float(np.max(np.abs(np.array(e) - e[0])))}
+    with open(a.out, "w") as fh:
+        json.dump(out, fh, indent=0)
+    print(f"slope {slope:.4e}  written {a.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,45 @@
+"""Energy conservation of the GPU exact path against the FP64 run of the same system.
+
+tests/golden/drift_fp64_fcc4000_atm.json holds E_total / N every 100 steps of a 10,000-step
+NVE run of 4,000 argon atoms (FCC 10^3 x 4, rho 0.8, T* 0.8, LJ + ATM over every triplet
+within rc) computed entirely in FP64 by the oracle (tests/golden/make_drift_fp64.py).  Both
+drifts are dominated by the energy steps of pairs and triplets crossing the unshifted
+cutoff (LJ12_6_Sensor.cpp:76), which make the slope a noisy quantity: the bars allow 40 %.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import komd_amd as K
+from komd_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "drift_fp64_fcc4000_atm.json")
+
+
+def test_energy_drift_matches_fp64_run():
+    with open(FIXTURE) as fh:
+        ref = json.load(fh)
+    s = S.fcc(10, 0.8, 0.8, 1003)
+    steps = ref["steps"]
+    e = []
+    with K.Context(s, cutoff=2.5, dt=0.005, nu=0.073) as c:
+        c.compute_forces(K.ATM_EXACT)
+        for k, st in enumerate(steps):
+            if k:
+                c.run(st - steps[k - 1], kind3b=K.ATM_EXACT)
+            o = c.observables(K.ATM_EXACT)
+            e.append((o["kinetic"] + o["u_lj"] + o["u_3b"]) / s.n)
+    e = np.array(e)
+    eref = np.array(ref["e_per_atom"])
+    t = np.array(steps) * 0.005
+    slope = np.polyfit(t, e, 1)[0]
+    print(f"E0 gpu {e[0]:.9f} fp64 {eref[0]:.9f}; slope gpu {slope:.3e} fp64 {ref['slope_e_per_atom_per_tau']:.3e}")
+    # same start: FP32 pair/triplet energies summed in FP64
+    assert abs(e[0] - eref[0]) < 5e-6
+    # a step whose energy jumped away would show up here (FP64: max deviation 0.014 per atom)
+    assert np.abs(e - e[0]).max() < 1.4 * ref["max_dev_per_atom"]
+    assert abs(slope - ref["slope_e_per_atom_per_tau"]) < 0.4 * abs(ref["slope_e_per_atom_per_tau"])

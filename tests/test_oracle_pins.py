@@ -314,3 +314,20 @@ def test_limit_clamps_components():
     f = np.array([[100.0, -100.0, 3.0]] * s.n)
     g = O.limit(s, f, 20.0)
     assert np.all(g[:, 0] == 80.0) and np.all(g[:, 1] == -80.0) and np.all(g[:, 2] == 3.0)
+
+
+def test_drift_fixture_consistent():
+    """The stored FP64 energy trajectory (tests/golden/make_drift_fp64.py) starts at the
+    oracle's energy of the generated state, and its slope is the fit of its own samples."""
+    import json
+    with open(os.path.join(FIXTURES, "drift_fp64_fcc4000_atm.json")) as fh:
+        ref = json.load(fh)
+    s = S.fcc(10, 0.8, 0.8, 1003)
+    r, b = O.update_container(s)
+    _, u, _ = O.lj_c08(s, r, b, energy=True, nthreads=4)
+    _, ea, _ = O.atm_complete(s, r, b, NU, coverage=0, sign=-1.0, nthreads=4)
+    e0 = (0.5 * float(np.sum(s.mass[:, None] * s.v * s.v)) + u + ea) / s.n
+    assert abs(e0 - ref["e_per_atom"][0]) < 1e-12
+    t = np.array(ref["steps"]) * 0.005
+    assert abs(np.polyfit(t, np.array(ref["e_per_atom"]), 1)[0] - ref["slope_e_per_atom_per_tau"]) < 1e-12
+    assert ref["steps"][-1] == 10000 and len(ref["steps"]) == 101
