@@ -1,11 +1,12 @@
 // Cell-tile pair kernel (LJ 12-6, force clamp, full neighbour lists) and the
 // per-site Axilrod-Teller-Muto kernel -- gfx950.
 //
-// k_cell: one 256-thread workgroup per domain cell ("home cell").
+// k_cell: one 256-thread workgroup per domain cell ("home cell"); on grids below
+// 1024 cells, `split` workgroups share a cell's home sites (about 5 each).
 //  * The 27 neighbour cells' sites are staged in LDS (in chunks) as FP32
 //    positions relative to the home cell centre, periodic images folded into
 //    the staging offset.
-//  * Every home site is served by 4-16 consecutive lanes striding over the
+//  * Every home site is served by 4-32 consecutive lanes striding over the
 //    staged candidates.  FP32 distance test; a candidate within a few ulps of
 //    rc^2 is decided in FP64 with the reference's operation order, so the
 //    interaction set is the reference's.  LJ force summed per lane in FP64 and
@@ -70,7 +71,7 @@ struct CellArgs {
     double* __restrict__ vz;
     const double* __restrict__ mass;
     Grid g;
-    int chunk, nblocks;
+    int chunk, nblocks, split;
     float rc2f, band, sig2, eps24, eps4;
     double rc2, limit, kick;
 };
@@ -130,7 +131,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int b = blockIdx.x;
-    const int cell = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: each XCD gets a contiguous slab
+    const int lb = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: each XCD gets a contiguous slab
+    const int cell = lb / a.split, part = lb - cell * a.split;  // part: share of the cell's home sites
     double u_acc = 0.0, w_acc = 0.0;
     double vir[kSym] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // pair virial r_ij (x) F_ij (energy mode)
     unsigned long long npairs = 0;
@@ -171,8 +173,13 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
         m = __builtin_amdgcn_readfirstlane(nb_pref[27]);
         hs = __builtin_amdgcn_readfirstlane(nb_start[13]);
         nh = __builtin_amdgcn_readfirstlane(nb_pref[14] - nb_pref[13]);
+        // this workgroup's share of the home sites
+        const int nhp = (nh + a.split - 1) / a.split;
+        const int hb = min(nh, part * nhp);
+        hs += hb;
+        nh = min(nh - hb, nhp);
     }
-    const int lps = nh <= 16 ? 16 : (nh <= 32 ? 8 : 4);
+    const int lps = nh <= 8 ? 32 : (nh <= 16 ? 16 : (nh <= 32 ? 8 : 4));
     const int per = kCellThreads / lps;
     const int h = tid / lps, sub = tid % lps;
     // the lps lanes of one home site are consecutive lanes of one wave
@@ -585,7 +592,7 @@ hipError_t dispatch(komd_ctx* c, const CellArgs& a, bool lj, int nl, bool energy
 
 }  // namespace
 
-int cell_blocks(komd_ctx* c) { return 8 * ((c->grid.ncell + 7) / 8); }
+int cell_blocks(komd_ctx* c) { return 8 * ((c->grid.ncell * c->cell_split + 7) / 8); }
 int atm_blocks(komd_ctx* c) { return 8 * (((int)((c->n + kAtmWaves - 1) / kAtmWaves) + 7) / 8); }
 
 hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
@@ -597,7 +604,8 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
     a.nl = d.nl; a.nlc = d.nlc; a.nld = d.nld;
     a.part = d.red; a.vpart = d.vpart; a.flags = d.flags; a.counters = d.cnt_pair;
     a.g = c->grid;
-    a.chunk = (c->grid.ncell + 7) / 8;
+    a.split = c->cell_split;
+    a.chunk = (c->grid.ncell * a.split + 7) / 8;
     a.nblocks = 8 * a.chunk;
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;

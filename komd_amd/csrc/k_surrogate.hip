@@ -1399,7 +1399,7 @@ hipError_t hybrid_forces(komd_ctx* c, bool energy) {
 }
 
 int surrogate_energy_blocks(komd_ctx* c) { return mlp_blocks(c) + 1; }
-int surrogate_virial_blocks(komd_ctx* c) { return cell_blocks(c) + 1; }
+int surrogate_virial_blocks(komd_ctx* c) { return 8 * ((c->grid.ncell + 7) / 8) + 1; }  // k_sur_force blocks + 1
 
 hipError_t surrogate_prepare(komd_ctx* c, bool hybrid) {
     Surrogate* s = c->sur;

@@ -356,6 +356,13 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     std::vector<double> buf((size_t)n);
     const size_t nb = (size_t)n * sizeof(double);
     const size_t C = (size_t)c->grid.ncell;
+    // Small grids cannot fill the GPU with one workgroup per cell: the pair pass then
+    // splits each cell's home sites over several workgroups of about 5 sites.
+    {
+        const int64_t C0 = c->grid.ncell;
+        const int64_t mean = (n + C0 - 1) / std::max<int64_t>(C0, 1);
+        c->cell_split = C0 >= 1024 ? 1 : (int)std::min<int64_t>(8, std::max<int64_t>(1, (mean + 4) / 5));
+    }
     const int pair_blocks = cell_blocks(c);
     c->red_blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
     const size_t red_n = std::max<size_t>({(size_t)2 * pair_blocks, (size_t)atm_blocks(c), (size_t)3 * c->red_blocks,

@@ -109,6 +109,7 @@ struct komd_ctx {
     bool have_sites = false;
     bool nl_valid = false;
     int red_blocks = 0;
+    int cell_split = 1;        // workgroups sharing one home cell in the pair pass (small grids)
     // timing
     bool timing = false;
     std::unordered_map<std::string, komd::Timer> timers;
