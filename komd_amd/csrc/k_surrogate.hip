@@ -221,74 +221,136 @@ __device__ __forceinline__ void win_gauss(float e, float g[kWin]) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_sur_desc(DescArgs a) {
-    const int tid = blockIdx.x * 256 + threadIdx.x;
-    const int p = tid / kLps, sub = tid % kLps;
-    if (p >= a.n) return;
-    float R[NR], M[NT];
+// One workgroup per 32 sites.  Waves 0-1 (4 lanes per site) accumulate the 76 moment
+// sums and write the angular invariants and T; waves 2-3 (4 lanes per site) accumulate
+// the 32 radial sums.  Each lane then holds one of the two accumulator sets only, which
+// halves the registers of the single-role form and doubles its occupancy.
+constexpr int kDescSites = 32;
+// LDS writes of this wave visible to its later reads
+__device__ __forceinline__ void desc_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+template <bool ANG>
+__device__ __forceinline__ void desc_role(const DescArgs& a, int p, int sub, float* __restrict__ lds) {
+    constexpr int NACC = ANG ? NT : NR;
+    float acc[NACC];
 #pragma unroll
-    for (int k = 0; k < NR; k++) R[k] = 0.f;
-#pragma unroll
-    for (int k = 0; k < NT; k++) M[k] = 0.f;
-    int nn = (int)a.nlc[p];
-    if (a.lam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // weight 1 - lambda = 0: descriptors unused
-    const float4* row = a.nl + (size_t)p * kFullCap;
+    for (int k = 0; k < NACC; k++) acc[k] = 0.f;
+    int nn = 0;
+    if (p < a.n) {
+        nn = (int)a.nlc[p];
+        if (a.lam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // weight 1 - lambda = 0: descriptors unused
+    }
+    const float4* row = a.nl + (size_t)(p < a.n ? p : 0) * kFullCap;
     // neighbour entries are loaded one iteration ahead of their use
     float4 vnext = sub < nn ? row[sub] : make_float4(1.f, 0.f, 0.f, 0.f);
     for (int s = sub; s < nn; s += kLps) {
         const float4 v = vnext;
         if (s + kLps < nn) vnext = row[s + kLps];
         const Pair q = pair_of(v, a.pirc);
-        float g[NR];
-        radial_gauss((q.r - kRMu0) * kRInvW, g);
+        if (ANG) {
+            float t[NC];
+            comps(q.ux, q.uy, q.uz, t);
 #pragma unroll
-        for (int k = 0; k < NR; k++) R[k] += g[k] * q.fc;
-        float t[NC];
-        comps(q.ux, q.uy, q.uz, t);
+            for (int ch = 0; ch < NA; ch++) {
+                const float x = (q.r - a_mu(ch)) * kAInvW;
+                const float psi = __expf(-x * x) * q.fc * (q.ir * q.ir * q.ir);
 #pragma unroll
-        for (int ch = 0; ch < NA; ch++) {
-            float psi, dpsi;
-            ang_basis(q, ch, psi, dpsi);
+                for (int c = 0; c < NC; c++) acc[ch * NC + c] += psi * t[c];
+            }
+        } else {
+            float g[NR];
+            radial_gauss((q.r - kRMu0) * kRInvW, g);
 #pragma unroll
-            for (int c = 0; c < NC; c++) M[ch * NC + c] += psi * t[c];
+            for (int k = 0; k < NR; k++) acc[k] += g[k] * q.fc;
         }
     }
+    if (ANG) {
+        // the 4 lanes' moment sums combined through LDS (a register reduction of 76 values
+        // keeps a temporary per value): every lane stores its sums, each lane adds a
+        // quarter of the components over the 4 lanes (fixed order), every lane reads the
+        // totals back
+        const int l = threadIdx.x & 127;
+        float* mine = lds + l * (NT + 1);
+        float* site = lds + (l & ~3) * (NT + 1);
 #pragma unroll
-    for (int o = 1; o < kLps; o <<= 1) {
+        for (int k = 0; k < NT; k++) mine[k] = acc[k];
+        desc_wave_sync();
+        float tot[(NT + kLps - 1) / kLps];
 #pragma unroll
-        for (int k = 0; k < NR; k++) R[k] += __shfl_xor(R[k], o);
+        for (int q = 0; q < (NT + kLps - 1) / kLps; q++) {
+            const int k = q * kLps + sub;
+            tot[q] = k < NT ? ((site[k] + site[(NT + 1) + k]) + site[2 * (NT + 1) + k]) + site[3 * (NT + 1) + k]
+                            : 0.f;
+        }
+        desc_wave_sync();
 #pragma unroll
-        for (int k = 0; k < NT; k++) M[k] += __shfl_xor(M[k], o);
+        for (int q = 0; q < (NT + kLps - 1) / kLps; q++) {
+            const int k = q * kLps + sub;
+            if (k < NT) site[k] = tot[q];
+        }
+        desc_wave_sync();
+    } else {
+#pragma unroll
+        for (int o = 1; o < kLps; o <<= 1) {
+#pragma unroll
+            for (int k = 0; k < NACC; k++) acc[k] += __shfl_xor(acc[k], o);
+        }
     }
-    float D[ND];
-#pragma unroll
-    for (int k = 0; k < NR; k++) D[k] = R[k];
-#pragma unroll
-    for (int m = 0; m < 16; m++) {
-        int ia, ib, L;
-        inv_of(m, ia, ib, L);
-        float sm = 0.f;
-#pragma unroll
-        for (int c = 0; c < 10; c++)
-            if (c < l_cnt(L)) {
-                const int cc = l_off(L) + c;
-                sm += mult(cc) * M[ia * NC + cc] * M[ib * NC + cc];
-            }
-        D[NR + m] = sm;
-    }
-    if (a.params) {
-#pragma unroll
-        for (int k = 0; k < ND; k++) D[k] = (D[k] - a.params[O_SHIFT + k]) * a.params[O_SCALE + k];
-    }
-    // the lanes of the site share the stores
+    if (p >= a.n) return;
     float4* xo = reinterpret_cast<float4*>(a.X + (size_t)p * ND);
+    if (ANG) {
+        // invariants and T from the site's summed moments in LDS (same address in its 4 lanes)
+        const float* M = lds + ((threadIdx.x & 127) & ~3) * (NT + 1);
+        float D[16];
 #pragma unroll
-    for (int k = 0; k < ND / 4; k++)
-        if (k % kLps == sub) xo[k] = make_float4(D[4 * k], D[4 * k + 1], D[4 * k + 2], D[4 * k + 3]);
-    float4* to = reinterpret_cast<float4*>(a.T + (size_t)p * NT);
+        for (int m = 0; m < 16; m++) {
+            int ia, ib, L;
+            inv_of(m, ia, ib, L);
+            float sm = 0.f;
 #pragma unroll
-    for (int k = 0; k < NT / 4; k++)
-        if (k % kLps == sub) to[k] = make_float4(M[4 * k], M[4 * k + 1], M[4 * k + 2], M[4 * k + 3]);
+            for (int c = 0; c < 10; c++)
+                if (c < l_cnt(L)) {
+                    const int cc = l_off(L) + c;
+                    sm += mult(cc) * M[ia * NC + cc] * M[ib * NC + cc];
+                }
+            D[m] = sm;
+        }
+        if (a.params) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) D[m] = (D[m] - a.params[O_SHIFT + NR + m]) * a.params[O_SCALE + NR + m];
+        }
+        // the lanes of the site share the stores: invariants are float4 chunks 8..11 of the row
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k == sub) xo[8 + k] = make_float4(D[4 * k], D[4 * k + 1], D[4 * k + 2], D[4 * k + 3]);
+        float4* to = reinterpret_cast<float4*>(a.T + (size_t)p * NT);
+#pragma unroll
+        for (int k = 0; k < NT / 4; k++)
+            if (k % kLps == sub) to[k] = make_float4(M[4 * k], M[4 * k + 1], M[4 * k + 2], M[4 * k + 3]);
+    } else {
+        if (a.params) {
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = (acc[k] - a.params[O_SHIFT + k]) * a.params[O_SCALE + k];
+        }
+        // radial descriptors are float4 chunks 0..7 of the row
+#pragma unroll
+        for (int k = 0; k < NR / 4; k++)
+            if (k % kLps == sub) xo[k] = make_float4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]);
+    }
+}
+
+// 4 waves per SIMD (the LDS of the angular reduction allows 4 workgroups per CU)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sur_desc(DescArgs a) {
+    __shared__ float red[128 * (NT + 1)];  // moment sums of the angular waves' lanes
+    const int l = threadIdx.x & 127;
+    const int p = blockIdx.x * kDescSites + l / kLps, sub = l % kLps;
+    if (threadIdx.x < 128)
+        desc_role<true>(a, p, sub, red);
+    else
+        desc_role<false>(a, p, sub, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1287,7 +1349,7 @@ hipError_t launch_desc(komd_ctx* c, bool raw, const float* lam = nullptr) {
     ScopedTimer tm(c, "sur_desc");
     Surrogate* s = c->sur;
     DescArgs a{c->d.nl, c->d.nlc, raw ? nullptr : s->params, s->X, s->T, lam, c->d.sorted, (int)c->n, pirc_of(c)};
-    k_sur_desc<<<group_blocks(c), 256, 0, c->stream>>>(a);
+    k_sur_desc<<<(int)((c->n + kDescSites - 1) / kDescSites), 256, 0, c->stream>>>(a);
     return hipGetLastError();
 }
 
