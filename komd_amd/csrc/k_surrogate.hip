@@ -13,8 +13,8 @@
 //                the input-gradient backward g = dE/dx.  The hidden units are
 //                stored in a lane-matched order (kappa below) so that a layer's
 //                accumulator registers are already the next layer's B operand:
-//                no LDS round trip between layers.  Writes the per-site state
-//                S = (dE/dD_radial[32], dE/dT_a^L[4][20]).
+//                no LDS round trip between layers.  Writes the per-site gradient
+//                S = dE/dD (48, scaled; the force pass forms dE/dT_a^L from it and T).
 //   k_sur_force  one workgroup per cell: F = -dE/dr as a sum over directed pairs of
 //                c_i(d_ij) = dE_i/dd_ij from site i's own state, +c to i and -c to j,
 //                accumulated in LDS over the cell's 27-cell neighbourhood and written
@@ -51,7 +51,7 @@ constexpr int NC = 19;             // unique components of T^1, T^2, T^3
 constexpr int NT = NA * NC;        // 76 moment values per site
 constexpr int ND = 48;             // descriptors
 constexpr int NH = 64;             // hidden width
-constexpr int NS = 112;            // state row: 32 radial + 4 x 20 angular
+constexpr int NS = 112;            // staged state row of the force pass: 32 radial + 4 x 20 angular
 constexpr int O_W1 = 0, O_B1 = O_W1 + NH * ND, O_W2 = O_B1 + NH, O_B2 = O_W2 + NH * NH, O_W3 = O_B2 + NH,
               O_B3 = O_W3 + NH, O_SHIFT = O_B3 + 1, O_SCALE = O_SHIFT + ND, NPARAM = O_SCALE + ND;
 static_assert(NPARAM == KOMD_SUR_NPARAM, "parameter layout");
@@ -511,13 +511,14 @@ struct MlpArgs {
     const float* __restrict__ T;
     const uint32_t* __restrict__ sorted;
     const float* __restrict__ lam;   // hybrid: exact-ATM weight per site (site order), or nullptr
-    float* __restrict__ S;           // [n][112]
+    float* __restrict__ S;           // [n][48] dE/dD (scaled, weighted by the surrogate share)
     float* __restrict__ E;           // [n] per-site energy (cell-sorted), optional
     double* __restrict__ part;       // per-workgroup energy partials, optional
     int n;
 };
 
-__global__ __launch_bounds__(256) void k_sur_mlp(MlpArgs a) {
+// 2 waves per SIMD (a 512-workgroup persistent grid fills both)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sur_mlp(MlpArgs a) {
     __shared__ bf8 ops[NOPS];
     __shared__ double red[4];
     for (int e = threadIdx.x; e < NOPS; e += 256) ops[e] = a.ops[e];
@@ -554,42 +555,12 @@ __global__ __launch_bounds__(256) void k_sur_mlp(MlpArgs a) {
         for (int m = 0; m < 3; m++)
 #pragma unroll
             for (int r = 0; r < 4; r++) G[4 * m + r] = o.g[m][r] * sc[4 * m + r] * wgt;
-        // every lane of the site needs the 16 angular entries (block 2 of the 4 lanes)
-        float ga[16];
-#pragma unroll
-        for (int qq = 0; qq < 4; qq++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) ga[4 * qq + r] = __shfl(G[8 + r], si + 16 * qq);
+        // lane q holds descriptors 16 m + 4 q + r: float4 chunks q, 4 + q, 8 + q of the row
         if (ok) {
-            float* srow = a.S + (size_t)p * NS;
-            reinterpret_cast<float4*>(srow)[q] = make_float4(G[0], G[1], G[2], G[3]);
-            reinterpret_cast<float4*>(srow)[4 + q] = make_float4(G[4], G[5], G[6], G[7]);
-            // dE/dT_q^L: sum over invariants containing channel q of G * T_partner
-            const float4* tr = reinterpret_cast<const float4*>(a.T + (size_t)p * NT);
-            float Tm[NT];
+            float4* grow = reinterpret_cast<float4*>(a.S + (size_t)p * ND);
 #pragma unroll
-            for (int k = 0; k < NT / 4; k++) {
-                const float4 t = tr[k];
-                Tm[4 * k] = t.x; Tm[4 * k + 1] = t.y; Tm[4 * k + 2] = t.z; Tm[4 * k + 3] = t.w;
-            }
-            float A[20];
-#pragma unroll
-            for (int c = 0; c < 20; c++) A[c] = 0.f;
-#pragma unroll
-            for (int m = 0; m < 16; m++) {
-                int ia, ib, L;
-                inv_of(m, ia, ib, L);
-                const float w = (ia == q ? ga[m] : 0.f), w2 = (ib == q ? ga[m] : 0.f);
-#pragma unroll
-                for (int c = 0; c < 10; c++)
-                    if (c < l_cnt(L)) {
-                        const int cc = l_off(L) + c;
-                        A[cc] += w * Tm[ib * NC + cc] + w2 * Tm[ia * NC + cc];
-                    }
-            }
-            float4* ar = reinterpret_cast<float4*>(srow + NR + 20 * q);
-#pragma unroll
-            for (int k = 0; k < 5; k++) ar[k] = make_float4(A[4 * k], A[4 * k + 1], A[4 * k + 2], A[4 * k + 3]);
+            for (int m = 0; m < 3; m++)
+                grow[4 * m + q] = make_float4(G[4 * m], G[4 * m + 1], G[4 * m + 2], G[4 * m + 3]);
         }
     }
     if (a.part) {
@@ -622,7 +593,8 @@ struct ForceArgs {
     const uint32_t* __restrict__ sorted;
     const uint32_t* __restrict__ key;
     const uint32_t* __restrict__ start;
-    const float* __restrict__ S;
+    const float* __restrict__ S;      // [n][48] dE/dD of every site (cell-sorted)
+    const float* __restrict__ T;      // [n][76] moment tensors of every site (cell-sorted)
     const float* __restrict__ E;      // hybrid: per-site surrogate energies (cell-sorted)
     const float* __restrict__ dlam;   // hybrid: d lambda / d axis coordinate (site order)
     const float* __restrict__ lam;    // hybrid: lambda (site order)
@@ -704,9 +676,35 @@ __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
     const int m = nb_base[27], hs = nb_start[13], nh = nb_cnt[13], self = nb_base[13];
     float wv[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int e = tid; e < m; e += kSurForceThreads) acc[0][e] = acc[1][e] = acc[2][e] = 0.f;
-    {
-        const float4* src = reinterpret_cast<const float4*>(a.S + (size_t)hs * NS);
-        for (int e = tid; e < nh * (NS / 4); e += kSurForceThreads) hst4[e] = src[e];
+    // home-site state: dE/dD_radial (32) and, per angular channel q, the 20 slots of
+    // dE/dT_q = sum over the invariants <T_a^L, T_b^L> containing q of dE/dinv * T_partner
+    for (int e = tid; e < nh * (NR / 4); e += kSurForceThreads) {
+        const int h = e / (NR / 4), k4 = e % (NR / 4);
+        hst4[h * (NS / 4) + k4] = reinterpret_cast<const float4*>(a.S + (size_t)(hs + h) * ND)[k4];
+    }
+    for (int e = tid; e < nh * NA; e += kSurForceThreads) {
+        const int h = e / NA, q = e % NA;
+        const float* Tm = a.T + (size_t)(hs + h) * NT;
+        const float* ga = a.S + (size_t)(hs + h) * ND + NR;
+        float A[20];
+#pragma unroll
+        for (int c = 0; c < 20; c++) A[c] = 0.f;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            int ia, ib, L;
+            inv_of(m, ia, ib, L);
+            const float g = ga[m];
+            const float w = ia == q ? g : 0.f, w2 = ib == q ? g : 0.f;
+#pragma unroll
+            for (int c = 0; c < 10; c++)
+                if (c < l_cnt(L)) {
+                    const int cc = l_off(L) + c;
+                    A[cc] += w * Tm[ib * NC + cc] + w2 * Tm[ia * NC + cc];
+                }
+        }
+        float4* ar = hst4 + h * (NS / 4) + (NR + 20 * q) / 4;
+#pragma unroll
+        for (int k = 0; k < 5; k++) ar[k] = make_float4(A[4 * k], A[4 * k + 1], A[4 * k + 2], A[4 * k + 3]);
     }
     __syncthreads();
     const float* hst = reinterpret_cast<const float*>(hst4);
@@ -1330,7 +1328,7 @@ hipError_t ensure_site_buffers(komd_ctx* c) {
     const size_t n = (size_t)c->n;
     KOMD_CHECK_HIP(salloc(&s->X, n * ND));
     KOMD_CHECK_HIP(salloc(&s->T, n * NT));
-    KOMD_CHECK_HIP(salloc(&s->S, n * NS));
+    KOMD_CHECK_HIP(salloc(&s->S, n * ND));
     KOMD_CHECK_HIP(salloc(&s->E, n));
     KOMD_CHECK_HIP(salloc(&s->params, NPARAM));
     KOMD_CHECK_HIP(salloc(&s->ops, NOPS));
@@ -1339,7 +1337,7 @@ hipError_t ensure_site_buffers(komd_ctx* c) {
 
 int mlp_blocks(komd_ctx* c) {
     const int64_t tiles = (c->n + 15) / 16;
-    return (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 256));
+    return (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 512));
 }
 int site_blocks(komd_ctx* c) { return (int)((c->n + 255) / 256); }
 int group_blocks(komd_ctx* c) { return (int)((c->n * kLps + 255) / 256); }  // kLps lanes per site
@@ -1365,7 +1363,8 @@ ForceArgs force_args(komd_ctx* c) {
     Surrogate* s = c->sur;
     ForceArgs a{};
     a.nl = c->d.nl; a.nlc = c->d.nlc; a.nld = c->d.nld; a.inv = c->d.inv; a.sorted = c->d.sorted;
-    a.key = c->d.key; a.start = c->d.start; a.S = s->S; a.E = s->E; a.P = c->d.rows; a.flags = c->d.flags;
+    a.key = c->d.key; a.start = c->d.start; a.S = s->S; a.T = s->T; a.E = s->E; a.P = c->d.rows;
+    a.flags = c->d.flags;
     a.g = c->grid;
     a.chunk = (c->grid.ncell + 7) / 8;
     a.fx = c->d.fx; a.fy = c->d.fy; a.fz = c->d.fz;
