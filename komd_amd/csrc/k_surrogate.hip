@@ -33,6 +33,7 @@
 #include <cmath>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "komd_cell.hpp"
@@ -630,15 +631,31 @@ __device__ __forceinline__ int wrap_cell(const Grid& g, int x, int y, int z) {
 // window of a pair from the staged state in LDS.  c = dE_i/dd_ij: +c is summed in
 // registers and combined inside the lane group once per site, -c goes to j's slot of
 // the LDS accumulator (cell direction and rank carried by the list entry).
+// The accumulator is fixed point (kFixScale per unit, 32-bit LDS integer adds, which
+// run far faster than LDS float adds and make the sums independent of their order).
+// Every add is below kFixMax in magnitude and a slot takes at most kRows + 1 adds (one
+// per home site listing it, plus its own sum when it is a home site), so no sum can
+// wrap.  A block that meets a larger or non-finite value (each lane folds the range test
+// into a flag, the adds themselves run unconditionally) sets `big` and runs its pairs
+// again with float adds.
 // VIR: also the virial sum over directed pairs of -(r_j - r_i) (x) c (c is the force on i
 // from E_i through the pair; -c acts on j), per-block partials in a.vpart.
 constexpr int kSurForceThreads = 256;
+constexpr float kFixScale = 1048576.f;  // 2^20
+constexpr float kFixMax = 16.f;         // kFixMax * kFixScale = 2^24
+static_assert((kRows + 1) * (double)kFixMax * kFixScale < 2147483648.0, "fixed-point slot sum can wrap");
+__device__ __forceinline__ bool fix_ok(float x, float y, float z) {
+    return fabsf(x) < kFixMax && fabsf(y) < kFixMax && fabsf(z) < kFixMax;  // false for NaN
+}
 template <bool VIR>
 __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
     __shared__ int nb_start[27], nb_cnt[27], nb_base[28];
-    __shared__ float acc[3][kSurM];
+    __shared__ union {
+        int i[3][kSurM];
+        float f[3][kSurM];
+    } acc;
     __shared__ float4 hst4[kRows * NS / 4];
-    __shared__ int bad;
+    __shared__ int bad, big;
     const int tid = threadIdx.x, b = blockIdx.x;
     const int cell = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous slab of cells per XCD
     if (cell >= a.g.ncell) {
@@ -665,6 +682,7 @@ __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
         if (tid == 0) {
             nb_base[0] = 0;
             bad = over;
+            big = 0;
             if (over) atomicOr(a.flags, FLAG_CELL_OVERFLOW);
         }
     }
@@ -674,8 +692,8 @@ __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
         return;
     }
     const int m = nb_base[27], hs = nb_start[13], nh = nb_cnt[13], self = nb_base[13];
-    float wv[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int e = tid; e < m; e += kSurForceThreads) acc[0][e] = acc[1][e] = acc[2][e] = 0.f;
+    float wv[kSym];
+    for (int e = tid; e < m; e += kSurForceThreads) acc.i[0][e] = acc.i[1][e] = acc.i[2][e] = 0;
     // home-site state: dE/dD_radial (32) and, per angular channel q, the 20 slots of
     // dE/dT_q = sum over the invariants <T_a^L, T_b^L> containing q of dE/dinv * T_partner
     for (int e = tid; e < nh * (NR / 4); e += kSurForceThreads) {
@@ -710,90 +728,126 @@ __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
     const float* hst = reinterpret_cast<const float*>(hst4);
     const int lps = nh <= 16 ? 16 : (nh <= 32 ? 8 : 4);
     const int per = kSurForceThreads / lps;
-    for (int h0 = 0; h0 < nh; h0 += per) {
-        const int h = h0 + tid / lps, sub = tid % lps;
-        const bool live = h < nh;
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        if (live) {
-            const float* si = hst + h * NS;
-            float A[80];
-            {
-                const float4* ar = reinterpret_cast<const float4*>(si + NR);
+    // one pass over the home sites' pairs; FIXED: fixed-point adds, else float adds
+    auto pairs = [&](auto fixed_tag) {
+        constexpr bool fixed = decltype(fixed_tag)::value;
+        bool ok = true;
 #pragma unroll
-                for (int k = 0; k < 20; k++) {
-                    const float4 t4 = ar[k];
-                    A[4 * k] = t4.x; A[4 * k + 1] = t4.y; A[4 * k + 2] = t4.z; A[4 * k + 3] = t4.w;
+        for (int k = 0; k < kSym; k++) wv[k] = 0.f;
+        for (int h0 = 0; h0 < nh; h0 += per) {
+            const int h = h0 + tid / lps, sub = tid % lps;
+            const bool live = h < nh;
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            if (live) {
+                const float* si = hst + h * NS;
+                float A[80];
+                {
+                    const float4* ar = reinterpret_cast<const float4*>(si + NR);
+#pragma unroll
+                    for (int k = 0; k < 20; k++) {
+                        const float4 t4 = ar[k];
+                        A[4 * k] = t4.x; A[4 * k + 1] = t4.y; A[4 * k + 2] = t4.z; A[4 * k + 3] = t4.w;
+                    }
+                }
+                const int p = hs + h;
+                int nn = (int)a.nlc[p];
+                if (a.dlam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // hybrid: no surrogate energy on this site
+                const float4* row = a.nl + (size_t)p * kFullCap;
+                const uint16_t* drow = a.nld + (size_t)p * kFullCap;
+                // the next entry is loaded one iteration ahead of its use
+                float4 vnext = sub < nn ? row[sub] : make_float4(1.f, 0.f, 0.f, 0.f);
+                uint32_t dnext = sub < nn ? drow[sub] : 0u;
+                for (int s = sub; s < nn; s += lps) {
+                    const float4 v = vnext;
+                    const uint32_t dr = dnext;
+                    if (s + lps < nn) {
+                        vnext = row[s + lps];
+                        dnext = drow[s + lps];
+                    }
+                    const Pair q = pair_of(v, a.pirc);
+                    // radial: sum_k S_k d(G_k fc)/dr = dfc sa - 2 fc / w ((y - k0) sa - sc),
+                    // sa = sum_t S_{k0+t} g_t, sc = sum_t t S_{k0+t} g_t
+                    const float y = (q.r - kRMu0) * kRInvW;
+                    const int k0 = win_start(y);
+                    const float ew = y - (float)k0;
+                    float g[kWin];
+                    win_gauss(ew, g);
+                    float sa = 0.f, sc = 0.f;
+#pragma unroll
+                    for (int t = 0; t < kWin; t++) {
+                        const float w = si[k0 + t] * g[t];
+                        sa += w;
+                        sc += (float)t * w;
+                    }
+                    const float coef = q.dfc * sa - 2.f * kRInvW * q.fc * (ew * sa - sc);
+                    float cx = coef * q.ux, cy = coef * q.uy, cz = coef * q.uz;
+#pragma unroll
+                    for (int ch = 0; ch < NA; ch++) {
+                        float psi, dpsi;
+                        ang_basis(q, ch, psi, dpsi);
+                        chan_grad(A + 20 * ch, q.ux, q.uy, q.uz, dpsi, psi * q.ir, cx, cy, cz);
+                    }
+                    fx += cx;
+                    fy += cy;
+                    fz += cz;
+                    if (VIR) sym_outer<float>(wv, -v.x, -v.y, -v.z, cx, cy, cz);
+                    const int d = (int)(dr & 31u), rk = (int)(dr >> 5);
+                    if (rk < nb_cnt[d]) {  // lists and cells come from one binning
+                        const int slot = nb_base[d] + rk;
+                        if constexpr (!fixed) {
+                            atomicAdd(&acc.f[0][slot], -cx);
+                            atomicAdd(&acc.f[1][slot], -cy);
+                            atomicAdd(&acc.f[2][slot], -cz);
+                        } else {  // out of range: the sums of this pass are discarded
+                            ok = ok & fix_ok(cx, cy, cz);
+                            atomicAdd(&acc.i[0][slot], __float2int_rn(-cx * kFixScale));
+                            atomicAdd(&acc.i[1][slot], __float2int_rn(-cy * kFixScale));
+                            atomicAdd(&acc.i[2][slot], __float2int_rn(-cz * kFixScale));
+                        }
+                    }
                 }
             }
-            const int p = hs + h;
-            int nn = (int)a.nlc[p];
-            if (a.dlam && a.lam[a.sorted[p]] == 1.f) nn = 0;  // hybrid: no surrogate energy on this site
-            const float4* row = a.nl + (size_t)p * kFullCap;
-            const uint16_t* drow = a.nld + (size_t)p * kFullCap;
-            // the next entry is loaded one iteration ahead of its use
-            float4 vnext = sub < nn ? row[sub] : make_float4(1.f, 0.f, 0.f, 0.f);
-            uint32_t dnext = sub < nn ? drow[sub] : 0u;
-            for (int s = sub; s < nn; s += lps) {
-                const float4 v = vnext;
-                const uint32_t dr = dnext;
-                if (s + lps < nn) {
-                    vnext = row[s + lps];
-                    dnext = drow[s + lps];
-                }
-                const Pair q = pair_of(v, a.pirc);
-                // radial: sum_k S_k d(G_k fc)/dr = dfc sa - 2 fc / w ((y - k0) sa - sc),
-                // sa = sum_t S_{k0+t} g_t, sc = sum_t t S_{k0+t} g_t
-                const float y = (q.r - kRMu0) * kRInvW;
-                const int k0 = win_start(y);
-                const float ew = y - (float)k0;
-                float g[kWin];
-                win_gauss(ew, g);
-                float sa = 0.f, sc = 0.f;
-#pragma unroll
-                for (int t = 0; t < kWin; t++) {
-                    const float w = si[k0 + t] * g[t];
-                    sa += w;
-                    sc += (float)t * w;
-                }
-                const float coef = q.dfc * sa - 2.f * kRInvW * q.fc * (ew * sa - sc);
-                float cx = coef * q.ux, cy = coef * q.uy, cz = coef * q.uz;
-#pragma unroll
-                for (int ch = 0; ch < NA; ch++) {
-                    float psi, dpsi;
-                    ang_basis(q, ch, psi, dpsi);
-                    chan_grad(A + 20 * ch, q.ux, q.uy, q.uz, dpsi, psi * q.ir, cx, cy, cz);
-                }
-                fx += cx;
-                fy += cy;
-                fz += cz;
-                if (VIR) sym_outer<float>(wv, -v.x, -v.y, -v.z, cx, cy, cz);
-                const int d = (int)(dr & 31u), rk = (int)(dr >> 5);
-                if (rk < nb_cnt[d]) {  // lists and cells come from one binning
-                    const int slot = nb_base[d] + rk;
-                    atomicAdd(&acc[0][slot], -cx);
-                    atomicAdd(&acc[1][slot], -cy);
-                    atomicAdd(&acc[2][slot], -cz);
+            for (int o = 1; o < lps; o <<= 1) {
+                fx += __shfl_xor(fx, o);
+                fy += __shfl_xor(fy, o);
+                fz += __shfl_xor(fz, o);
+            }
+            if (live && sub == 0) {
+                if constexpr (!fixed) {
+                    atomicAdd(&acc.f[0][self + h], fx);
+                    atomicAdd(&acc.f[1][self + h], fy);
+                    atomicAdd(&acc.f[2][self + h], fz);
+                } else {
+                    ok = ok & fix_ok(fx, fy, fz);
+                    atomicAdd(&acc.i[0][self + h], __float2int_rn(fx * kFixScale));
+                    atomicAdd(&acc.i[1][self + h], __float2int_rn(fy * kFixScale));
+                    atomicAdd(&acc.i[2][self + h], __float2int_rn(fz * kFixScale));
                 }
             }
         }
-        for (int o = 1; o < lps; o <<= 1) {
-            fx += __shfl_xor(fx, o);
-            fy += __shfl_xor(fy, o);
-            fz += __shfl_xor(fz, o);
-        }
-        if (live && sub == 0) {
-            atomicAdd(&acc[0][self + h], fx);
-            atomicAdd(&acc[1][self + h], fy);
-            atomicAdd(&acc[2][self + h], fz);
+        if (!ok) big = 1;
+    };
+    pairs(std::true_type{});
+    __syncthreads();
+    if (big) {  // block-uniform: read after the barrier, written only in the fixed pass
+        for (int e = tid; e < m; e += kSurForceThreads) acc.f[0][e] = acc.f[1][e] = acc.f[2][e] = 0.f;
+        __syncthreads();
+        pairs(std::false_type{});
+        __syncthreads();
+    }
+    // the rows out in one flat pass over (direction, rank); a wave covers one direction
+    static_assert(kRows == 64 && kSurForceThreads % kRows == 0, "one wave per direction row");
+    float4* out = a.P + (size_t)cell * 27 * kRows;
+    const bool as_int = !big;
+    for (int idx = tid; idx < 27 * kRows; idx += kSurForceThreads) {
+        const int d = idx / kRows, r = idx % kRows;
+        if (r < nb_cnt[d]) {
+            const int e = nb_base[d] + r;
+            out[idx] = as_int ? make_float4((float)acc.i[0][e] * (1.f / kFixScale), (float)acc.i[1][e] * (1.f / kFixScale),
+                                            (float)acc.i[2][e] * (1.f / kFixScale), 0.f)
+                              : make_float4(acc.f[0][e], acc.f[1][e], acc.f[2][e], 0.f);
         }
     }
-    __syncthreads();
-    float4* out = a.P + (size_t)cell * 27 * kRows;
-    for (int d = 0; d < 27; d++)
-        for (int r = tid; r < nb_cnt[d]; r += kSurForceThreads) {
-            const int e = nb_base[d] + r;
-            out[d * kRows + r] = make_float4(acc[0][e], acc[1][e], acc[2][e], 0.f);
-        }
     if (VIR) block_store_sym(a.vpart, a.vparts, b, wv);
 }
 

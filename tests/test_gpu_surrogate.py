@@ -90,6 +90,46 @@ def test_forces_match_oracle(cells, seed):
     assert abs(obs["u_3b"] - Eo) < 1e-2 * abs(Eo)
 
 
+def test_forces_bitwise_repeatable():
+    """The per-cell force rows are summed in fixed point, so two evaluations of one
+    state (each in a fresh context, forces starting from zero) give the same bits
+    whatever order the pair adds ran in."""
+    s = system(8, 15)
+    p = params_for(s, 4)
+    F = []
+    for _ in range(2):
+        with K.Context(s) as c:
+            c.surrogate_set_params(p)
+            c.update_container()
+            c.forces_3b(K.SURROGATE)
+            F.append(c.forces())
+    print(f"max |F| {np.abs(F[0]).max():.3e}", flush=True)
+    assert np.array_equal(F[0], F[1])
+
+
+def test_forces_large_scale_float_path():
+    """Final-layer weights scaled by 1e4 scale every force by 1e4, beyond the fixed-point
+    accumulator's per-add range: the blocks fall back to float adds and the result still
+    matches the oracle to the bar of the unscaled case."""
+    s = system(5, 16)
+    p = SO.unpack(params_for(s, 5).astype(np.float64))
+    p["w3"] = p["w3"] * 1e4
+    p = SO.pack(p).astype(np.float32)
+    with K.Context(s) as c:
+        c.surrogate_set_params(p)
+        c.update_container()
+        c.forces_3b(K.SURROGATE)
+        Fg = c.forces()
+    i, j, d = oracle_inputs(s)
+    _, _, Fo = SO.energy_forces(p.astype(np.float64), s.n, i, j, d)
+    scale = np.sqrt(np.mean(Fo ** 2))
+    dd = np.linalg.norm(Fg - Fo, axis=1)
+    rms = np.sqrt(np.mean(dd ** 2)) / scale
+    print(f"F rms scale {scale:.3e} rel rms {rms:.2e} max {dd.max() / scale:.2e}")
+    assert scale > 100.0  # site forces far beyond the per-add limit of 16 of the fixed-point sums
+    assert rms < 2e-2 and dd.max() / scale < 2e-1
+
+
 def test_training_gradient_matches_oracle():
     s = system(5, 15)
     p = params_for(s, 4)
