@@ -22,10 +22,11 @@
 // k_atm: one wave per site, four independent waves per workgroup.  The site's
 // list is loaded into registers (lane a holds x_a = r_a - r_i) and into LDS.
 // Pair filter in registers: y is rotated around the 64-lane circle with DPP
-// (wave_rol) together with its lane index, so shifts k = 1..31 for every lane
-// plus k = 32 for lanes < 32 visit every unordered neighbour pair once; pairs
-// with r_jk <= rc are compacted with a ballot into a 128-entry LDS ring and
-// evaluated 64 at a time without divergence.  Every site accumulates only its
+// (wave_rol), so shifts k = 1..31 for every lane plus k = 32 for lanes < 32
+// visit every unordered neighbour pair once; pairs with r_jk^2 within a small
+// band above rc^2 are compacted with a ballot into a 128-entry LDS ring as
+// (lane, k) and evaluated 64 at a time without divergence, the exact cutoff
+// decision taken there.  Every site accumulates only its
 // own three-body force, so there is no scatter and no atomic; each triplet is
 // visited by its three members and a visit carries V/3 of the energy.
 //
@@ -360,7 +361,7 @@ template <int ATM, bool ENERGY, bool HYB>
 __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     __shared__ float4 wbuf[kAtmWaves][kFullCap];
     __shared__ float2 wsq[kAtmWaves][kFullCap];  // |x_a|^2 and its reciprocal per neighbour
-    __shared__ uint32_t qbuf[kAtmWaves][128 + 64];  // ring of 128 pairs + one spill slot per lane
+    __shared__ uint32_t qbuf[kAtmWaves][128];  // ring of 128 candidate pairs
     __shared__ uint8_t wdir[ATM == 2 ? kAtmWaves : 1][ATM == 2 ? kFullCap : 1];
     __shared__ float wlam[HYB ? kAtmWaves : 1][HYB ? kFullCap : 1];
     __shared__ double red[kAtmWaves];
@@ -373,7 +374,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     const int p = lb * kAtmWaves + wvu;
     double e_acc = 0.0;
     float vir[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // triplet virial share of this lane's visits
-    unsigned long long nvisit = 0;
+    unsigned long long nvisit = 0;  // accepted visits of this wave (the same in every lane)
     bool p_skip = false;
     if (p < a.n) {
         const int nn = __builtin_amdgcn_readfirstlane((int)a.nlc[p]);
@@ -405,15 +406,29 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             // and the per-site energy output only
             const bool want_e = ENERGY || HYB || a.esite != nullptr;
             float fx = 0.f, fy = 0.f, fz = 0.f, ev = 0.f;
-            // dense evaluation of an in-range pair (x = r_j - r_i, y = r_k - r_i)
-            auto force = [&](int ia, int ib) {
+            // ring entries: a | b << 16 from the long-list filter, a | k << 16 (b = a + k mod 64)
+            // from the rotation filter (wave-uniform: lists of at most 64)
+            const bool rot = nn <= 64;
+            // dense evaluation of 64 candidate pairs (x = r_j - r_i, y = r_k - r_i), `live`
+            // lanes only.  The filters pass every pair with r_jk^2 <= rc^2 + band; the
+            // cutoff is decided here: FP32 r_jk^2 against rc^2, and within the band the
+            // FP64 distance with the reference's operation order (rare: wave-uniform
+            // branch), so the triplet set is the reference's.
+            auto force = [&](uint32_t e, bool live) {
+                const int ia = (int)(e & 0xFFFFu);
+                const int ib = rot ? (int)((e + (e >> 16)) & 63u) : (int)(e >> 16);
                 const float4 xv = wbuf[wvu][ia], yv = wbuf[wvu][ib];
                 const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
+                const float Z = zx * zx + zy * zy + zz * zz;
+                bool in = live && Z <= a.rc2f;
+                const bool amb = live && fabsf(Z - a.rc2f) <= a.band3;
+                if (__ballot(amb) != 0ull && amb)
+                    in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xv.w), __float_as_uint(yv.w)) > a.rc2);
+                nvisit += (unsigned long long)__popcll(__ballot(in));
                 float wt = 1.f;
                 if (HYB) wt = (li + wlam[wvu][ia] + wlam[wvu][ib]) * (1.f / 3.f);
                 const float2 sx = wsq[wvu][ia], sy = wsq[wvu][ib];
                 const float X = sx.x, Y = sy.x;
-                const float Z = zx * zx + zy * zy + zz * zz;
                 const float p1 = xv.x * yv.x + xv.y * yv.y + xv.z * yv.z;
                 const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
                 const float p3 = yv.x * zx + yv.y * zy + yv.z * zz;
@@ -422,42 +437,43 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 const float rD = __builtin_amdgcn_rsqf(D);
                 const float rD2 = rD * rD;
                 const float D52 = rD2 * rD2 * rD;
-                float K = 3.f * a.nu * D52 * wt;
+                // rejected and idle lanes select zeros (their D may be 0 or garbage)
+                float K = in ? 3.f * a.nu * D52 * wt : 0.f;
                 if (ATM == 2) K = -K;  // reference: +grad V
                 const float A = -(D + 5.f * Pp);
-                const float iX = sx.y, iY = sy.y;
                 const float t1 = p2 * p3, t23 = p1 * (p3 + p2);
-                const float gx = K * (A * (xv.x * iX + yv.x * iY) - t1 * (xv.x + yv.x) - t23 * zx);
-                const float gy = K * (A * (xv.y * iX + yv.y * iY) - t1 * (xv.y + yv.y) - t23 * zy);
-                const float gz = K * (A * (xv.z * iX + yv.z * iY) - t1 * (xv.z + yv.z) - t23 * zz);
-                fx += gx;
-                fy += gy;
-                fz += gz;
-                if (want_e) ev += a.nu * D52 * (D + 3.f * Pp);  // unweighted V of this visit
+                // with z = y - x the bracket is (A/X - t1 + t23) x + (A/Y - t1 - t23) y
+                const float cx = K * (A * sx.y - t1 + t23), cy = K * (A * sy.y - t1 - t23);
+                fx = fmaf(cy, yv.x, fmaf(cx, xv.x, fx));
+                fy = fmaf(cy, yv.y, fmaf(cx, xv.y, fy));
+                fz = fmaf(cy, yv.z, fmaf(cx, xv.z, fz));
+                if (want_e) ev += in ? a.nu * D52 * (D + 3.f * Pp) : 0.f;  // unweighted V of this visit
                 if (ENERGY) {
                     // (r_i - centroid) (x) F_i, r_i - centroid = -(x + y)/3; the three member
                     // visits sum to sum_m r_m (x) F_m.  Physical sign in both modes.
+                    const float gx = cx * xv.x + cy * yv.x, gy = cx * xv.y + cy * yv.y, gz = cx * xv.z + cy * yv.z;
                     const float sc = ATM == 2 ? (1.f / 3.f) : (-1.f / 3.f);
                     sym_outer<float>(vir, sc * (xv.x + yv.x), sc * (xv.y + yv.y), sc * (xv.z + yv.z), gx, gy, gz);
                 }
             };
             int head = 0, tail = 0;
-            // ballot-compact this instruction's in-range pairs into the ring; dense flush at 64
-            // (lanes without a pair write their own spill slot, so the store needs no branch)
-            auto push = [&](bool hit, int ia, int ib) {
+            // ballot-compact this instruction's candidates into the ring; dense flush at 64
+            auto push = [&](bool hit, uint32_t e) {
                 const unsigned long long bal = __ballot(hit);
-                const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                ring[hit ? ((tail + pos) & 127) : 128 + lane] = (uint32_t)ia | ((uint32_t)ib << 16);
+                if (hit) {
+                    const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    ring[(tail + pos) & 127] = e;
+                }
                 tail = __builtin_amdgcn_readfirstlane(tail + (int)__popcll(bal));
                 if (tail - head >= 64) {
                     wave_sync();
-                    const uint32_t e = ring[(head + lane) & 127];
-                    force((int)(e & 0xFFFFu), (int)(e >> 16));
+                    force(ring[(head + lane) & 127], true);
                     head += 64;
                     wave_sync();
                 }
             };
+            const float lim = a.rc2f + a.band3;  // candidate bound of the filters
             auto accept = [&](bool in, int ia, int ib) {
                 if (ATM == 2 && in) {
                     const int da = wdir[wvu][ia], db = wdir[wvu][ib];
@@ -469,29 +485,22 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 return in;
             };
             if (nn <= 64) {
-                // filter in registers: lane a holds x_a; after k DPP rotations (wave_rol)
-                // lane a holds y = x_{a-k mod 64} and its lane index yi
+                // filter in registers: lane a holds x_a; wave_rol moves lane a+1's value to
+                // lane a, so after k rotations lane a holds y = x_{a+k mod 64}
                 float yx = xa.x, yy = xa.y, yz = xa.z;
-                int yi = lane;
                 // wave_rol has a source lane for every lane: plain DPP moves, no old value
 #pragma unroll 2
                 for (int k = 1; k <= 32; k++) {
                     yx = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yx), 0x134, 0xF, 0xF, false));
                     yy = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yy), 0x134, 0xF, 0xF, false));
                     yz = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yz), 0x134, 0xF, 0xF, false));
-                    yi = __builtin_amdgcn_mov_dpp(yi, 0x134, 0xF, 0xF, false);
                     const float zx = yx - xa.x, zy = yy - xa.y, zz = yz - xa.z;
                     const float Z = zx * zx + zy * zy + zz * zz;
                     // lanes past the list hold distinct far-away fillers: no pair with them is in range
                     const bool valid = k < 32 || lane < 32;
-                    bool in = valid && Z <= a.rc2f;
-                    // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
-                    const bool amb = valid && fabsf(Z - a.rc2f) <= a.band3;
-                    if (__ballot(amb) != 0ull && amb)
-                        in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xa.w),
-                                        __float_as_uint(wbuf[wvu][yi].w)) > a.rc2);
-                    in = accept(in, lane, yi);
-                    push(in, lane, yi);
+                    bool in = valid && Z <= lim;
+                    if (ATM == 2 || HYB) in = accept(in, lane, (lane + k) & 63);
+                    push(in, (uint32_t)lane | ((uint32_t)k << 16));
                 }
             } else {
                 // long lists: circulant pairs (a, a+s+1 mod n) for s < H = (n-1)/2, plus
@@ -499,35 +508,29 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 const int H = (nn - 1) >> 1;
                 const int halfn = (nn & 1) ? 0 : (nn >> 1);
                 for (int sft = 0; sft <= H; sft++) {
-                    const int lim = sft < H ? nn : halfn;
-                    for (int a0 = 0; a0 < lim; a0 += 64) {
+                    const int lims = sft < H ? nn : halfn;
+                    for (int a0 = 0; a0 < lims; a0 += 64) {
                         const int ia = a0 + lane;
                         bool in = false;
                         int ib = 0;
-                        if (ia < lim) {
+                        if (ia < lims) {
                             const float4 xv = wbuf[wvu][ia];
                             ib = ia + sft + 1;
                             if (ib >= nn) ib -= nn;
                             const float4 yv = wbuf[wvu][ib];
                             const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
                             const float Z = zx * zx + zy * zy + zz * zz;
-                            if (fabsf(Z - a.rc2f) <= a.band3)
-                                in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xv.w), __float_as_uint(yv.w)) >
-                                       a.rc2);
-                            else
-                                in = Z <= a.rc2f;
-                            in = accept(in, ia, ib);
+                            in = accept(Z <= lim, ia, ib);
                         }
-                        push(in, ia, ib);
+                        push(in, (uint32_t)ia | ((uint32_t)ib << 16));
                     }
                 }
             }
             wave_sync();
-            if (lane < tail - head) {
-                const uint32_t e = ring[(head + lane) & 127];
-                force((int)(e & 0xFFFFu), (int)(e >> 16));
+            if (tail > head) {
+                const bool live = lane < tail - head;
+                force(live ? ring[(head + lane) & 127] : 0u, live);
             }
-            nvisit = (unsigned long long)tail;
             double dx = wave_sum((double)fx), dy = wave_sum((double)fy), dz = wave_sum((double)fz);
             const double de = want_e ? wave_sum((double)ev) * (1.0 / 3.0) : 0.0;  // this site's third of its triplets
             if (lane == 0) {
