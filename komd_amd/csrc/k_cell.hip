@@ -23,10 +23,11 @@
 // list is loaded into registers (lane a holds x_a = r_a - r_i) and into LDS.
 // Pair filter in registers: y is rotated around the 64-lane circle with DPP
 // (wave_rol), so shifts k = 1..31 for every lane plus k = 32 for lanes < 32
-// visit every unordered neighbour pair once; pairs with r_jk^2 within a small
-// band above rc^2 are compacted with a ballot into a 128-entry LDS ring as
-// (lane, k) and evaluated 64 at a time without divergence, the exact cutoff
-// decision taken there.  Every site accumulates only its
+// visit every unordered neighbour pair once (r_jk^2 from FP16 (x, y) pairs and
+// FP32 z, a 1 % candidate margin); the candidates are compacted with a ballot
+// into a 128-entry LDS ring as (lane, k) and evaluated 64 at a time without
+// divergence, the exact cutoff decision taken there.  Lists longer than 64
+// use a circulant walk over LDS in FP32.  Every site accumulates only its
 // own three-body force, so there is no scatter and no atomic; each triplet is
 // visited by its three members and a visit carries V/3 of the energy.
 //
@@ -350,11 +351,13 @@ struct AtmArgs {
     const double* __restrict__ mass;
     Grid g;
     int n, chunk, nblocks, hyb_axis;
+    int h16;                           // 1e-2 <= rc <= 1e4: the FP16 rotation filter is exact enough
     float rc2f, band3, nu;
     double rc2, kick;
 };
 
 constexpr int kAtmWaves = 4;
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 // ATM: 1 physical, 2 reference semantics.
 template <int ATM, bool ENERGY, bool HYB>
@@ -406,9 +409,10 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             // and the per-site energy output only
             const bool want_e = ENERGY || HYB || a.esite != nullptr;
             float fx = 0.f, fy = 0.f, fz = 0.f, ev = 0.f;
-            // ring entries: a | b << 16 from the long-list filter, a | k << 16 (b = a + k mod 64)
-            // from the rotation filter (wave-uniform: lists of at most 64)
-            const bool rot = nn <= 64;
+            // ring entries: a | b << 16 from the circulant filter, a | k << 16 (b = a + k mod 64)
+            // from the rotation filter (wave-uniform: lists of at most 64 and a cutoff in the
+            // FP16 filter's range)
+            const bool rot = nn <= 64 && a.h16;
             // dense evaluation of 64 candidate pairs (x = r_j - r_i, y = r_k - r_i), `live`
             // lanes only.  The filters pass every pair with r_jk^2 <= rc^2 + band; the
             // cutoff is decided here: FP32 r_jk^2 against rc^2, and within the band the
@@ -421,6 +425,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 const float zx = yv.x - xv.x, zy = yv.y - xv.y, zz = yv.z - xv.z;
                 const float Z = zx * zx + zy * zy + zz * zz;
                 bool in = live && Z <= a.rc2f;
+                const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
                 const bool amb = live && fabsf(Z - a.rc2f) <= a.band3;
                 if (__ballot(amb) != 0ull && amb)
                     in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xv.w), __float_as_uint(yv.w)) > a.rc2);
@@ -429,9 +434,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 if (HYB) wt = (li + wlam[wvu][ia] + wlam[wvu][ib]) * (1.f / 3.f);
                 const float2 sx = wsq[wvu][ia], sy = wsq[wvu][ib];
                 const float X = sx.x, Y = sy.x;
-                const float p1 = xv.x * yv.x + xv.y * yv.y + xv.z * yv.z;
-                const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
-                const float p3 = yv.x * zx + yv.y * zy + yv.z * zz;
+                const float p1 = X + p2, p3 = p2 + Z;  // y = x + z
                 const float D = X * Y * Z;
                 const float Pp = -(p1 * p2 * p3);
                 const float rD = __builtin_amdgcn_rsqf(D);
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                     wave_sync();
                 }
             };
-            const float lim = a.rc2f + a.band3;  // candidate bound of the filters
+            const float lim = a.rc2f + a.band3;  // candidate bound of the circulant filter
             auto accept = [&](bool in, int ia, int ib) {
                 if (ATM == 2 && in) {
                     const int da = wdir[wvu][ia], db = wdir[wvu][ib];
@@ -484,27 +487,40 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 if (HYB && in) in = (li + wlam[wvu][ia] + wlam[wvu][ib]) != 0.f;
                 return in;
             };
-            if (nn <= 64) {
+            if (rot) {
                 // filter in registers: lane a holds x_a; wave_rol moves lane a+1's value to
-                // lane a, so after k rotations lane a holds y = x_{a+k mod 64}
-                float yx = xa.x, yy = xa.y, yz = xa.z;
-                // wave_rol has a source lane for every lane: plain DPP moves, no old value
-#pragma unroll 2
-                for (int k = 1; k <= 32; k++) {
-                    yx = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yx), 0x134, 0xF, 0xF, false));
-                    yy = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yy), 0x134, 0xF, 0xF, false));
+                // lane a, so after k rotations lane a holds y = x_{a+k mod 64}.
+                // (x, y) as an FP16 pair, z in FP32; r_jk^2 from v_dot2_f32_f16 (exact FP16
+                // products, FP32 sum).  Rounding x and y to FP16 (unit roundoff u = 2^-11)
+                // moves r_jk^2 by at most 2|z|u(|x| + |y| + |z|) <= 6u rc^2 = 0.003 rc^2 near
+                // the cutoff, inside the 1 % margin, so every pair within rc is a candidate.
+                // Filler lanes convert to FP16 infinity: their r_jk^2 is inf or NaN, never a
+                // candidate.  wave_rol has a source lane for every lane: plain DPP moves.
+                const half2_t oxy = {(_Float16)xa.x, (_Float16)xa.y};
+                uint32_t mxy = __builtin_bit_cast(uint32_t, oxy);
+                float yz = xa.z;
+                const float limh = a.rc2f * 1.01f + a.band3;
+                auto step = [&](int k, bool valid) {
+                    mxy = (uint32_t)__builtin_amdgcn_mov_dpp((int)mxy, 0x134, 0xF, 0xF, false);
                     yz = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(yz), 0x134, 0xF, 0xF, false));
-                    const float zx = yx - xa.x, zy = yy - xa.y, zz = yz - xa.z;
-                    const float Z = zx * zx + zy * zy + zz * zz;
-                    // lanes past the list hold distinct far-away fillers: no pair with them is in range
-                    const bool valid = k < 32 || lane < 32;
-                    bool in = valid && Z <= lim;
+                    const half2_t d = __builtin_bit_cast(half2_t, mxy) - oxy;
+                    const float dz = yz - xa.z;
+                    const float Z = __builtin_amdgcn_fdot2(d, d, dz * dz, false);
+                    bool in = valid && Z <= limh;
                     if (ATM == 2 || HYB) in = accept(in, lane, (lane + k) & 63);
                     push(in, (uint32_t)lane | ((uint32_t)k << 16));
+                };
+#pragma clang loop unroll(disable)
+                for (int k = 1; k < 31; k += 2) {
+                    step(k, true);
+                    step(k + 1, true);
                 }
+                step(31, true);
+                step(32, lane < 32);  // the pairs (a, a + 32) once
             } else {
-                // long lists: circulant pairs (a, a+s+1 mod n) for s < H = (n-1)/2, plus
-                // s = H for a < n/2 when n is even, read from LDS
+                // long lists (and cutoffs outside the FP16 filter's range): circulant pairs
+                // (a, a+s+1 mod n) for s < H = (n-1)/2, plus s = H for a < n/2 when n is
+                // even, read from LDS, r_jk^2 in FP32
                 const int H = (nn - 1) >> 1;
                 const int halfn = (nn & 1) ? 0 : (nn >> 1);
                 for (int sft = 0; sft <= H; sft++) {
@@ -643,6 +659,7 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;
     a.band3 = 1e-5f * a.rc2f;
+    a.h16 = c->prm.cutoff >= 1e-2 && c->prm.cutoff <= 1e4;
     a.nu = (float)c->prm.nu;
     a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
     a.kick = (!energy && c->kick_in == KICK_IN_ATM) ? 0.5 * c->prm.delta_t : 0.0;

@@ -182,6 +182,25 @@ def test_atm_exact_forces(orc, cells, seed):
     assert abs(obs["u_3b"] - e_or) < 1e-5 * abs(e_or)
 
 
+def test_atm_exact_long_lists(orc):
+    """Density 1.1: an FCC site has 78 neighbours within 2.5 (shells of 12, 6, 24, 12
+    and 24), so every list is longer than 64 and takes the circulant LDS filter."""
+    s = S.fcc(5, 1.1, 0.8, 44, jitter=0.05)
+    print(f"atm long lists n={s.n}", flush=True)
+    r, b = orc.update_container(s)
+    f_or, e_or, hits = orc.atm_complete(s, r, b, NU, coverage=0, sign=-1.0, nthreads=8)
+    with K.Context(s, nu=NU) as c:
+        c.update_container()
+        c.counters(reset=True)
+        c.forces_3b(K.ATM_EXACT)
+        f = c.forces()
+        _, trip = c.counters()
+    assert trip == hits, "triplet set differs"
+    mx, rms = rel_err(f, f_or)
+    print(f"ATM long max {mx:.3e} rms {rms:.3e} triplets {hits}")
+    assert mx < 1e-4 and rms < 1e-5
+
+
 @pytest.mark.parametrize("cells,seed", [(5, 51), (10, 52)])
 def test_atm_reference_mode(orc, cells, seed):
     """Reference ATM_NOLIST semantics: +grad V on the C08 triplet set."""
