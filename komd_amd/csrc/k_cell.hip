@@ -352,6 +352,8 @@ struct AtmArgs {
     Grid g;
     int n, chunk, nblocks, hyb_axis;
     int h16;                           // 1e-2 <= rc <= 1e4: the FP16 rotation filter is exact enough
+    int lj;                            // also the LJ force of the site's list (uniform sites, no clamp)
+    float sig2, eps24;
     float rc2f, band3, nu;
     double rc2, kick;
 };
@@ -376,6 +378,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
     const int lb = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous site slabs per XCD
     const int p = lb * kAtmWaves + wvu;
     double e_acc = 0.0;
+    double ljx = 0.0, ljy = 0.0, ljz = 0.0;  // this lane's LJ pairs (a.lj)
     float vir[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // triplet virial share of this lane's visits
     unsigned long long nvisit = 0;  // accepted visits of this wave (the same in every lane)
     bool p_skip = false;
@@ -391,7 +394,18 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             const float4 v = row[s];
             wbuf[wvu][s] = v;
             const float X = v.x * v.x + v.y * v.y + v.z * v.z;
-            wsq[wvu][s] = make_float2(X, __builtin_amdgcn_rcpf(X));
+            const float iX = __builtin_amdgcn_rcpf(X);
+            wsq[wvu][s] = make_float2(X, iX);
+            if (!ENERGY && !HYB && a.lj) {
+                // LJ 12-6 of the pair (the pair pass's expression): force on i = -x fac,
+                // per-pair FP32 terms summed in FP64
+                const float s2 = a.sig2 * iX;
+                const float s6 = s2 * s2 * s2;
+                const float fac = a.eps24 * (2.f * s6 * s6 - s6) * iX;
+                ljx -= (double)(v.x * fac);
+                ljy -= (double)(v.y * fac);
+                ljz -= (double)(v.z * fac);
+            }
             if (s == lane) xa = v;
             if (ATM == 2) wdir[wvu][s] = (uint8_t)(a.nld[(size_t)p * kFullCap + s] & 31u);
             if (HYB) {
@@ -547,7 +561,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 const bool live = lane < tail - head;
                 force(live ? ring[(head + lane) & 127] : 0u, live);
             }
-            double dx = wave_sum((double)fx), dy = wave_sum((double)fy), dz = wave_sum((double)fz);
+            double dx = wave_sum((double)fx + ljx), dy = wave_sum((double)fy + ljy), dz = wave_sum((double)fz + ljz);
             const double de = want_e ? wave_sum((double)ev) * (1.0 / 3.0) : 0.0;  // this site's third of its triplets
             if (lane == 0) {
                 if (HYB) {  // -V grad(w): site i receives -(sum V/3) dl_i/dx_axis
@@ -645,7 +659,7 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
     return dispatch<false, true>(c, a, lj, nl, energy, lim);
 }
 
-hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite) {
+hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite, bool lj) {
     auto& d = c->d;
     AtmArgs a{};
     a.nl = d.nl; a.nlc = d.nlc; a.nld = d.nld; a.sorted = d.sorted;
@@ -660,6 +674,9 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     a.rc2f = (float)a.rc2;
     a.band3 = 1e-5f * a.rc2f;
     a.h16 = c->prm.cutoff >= 1e-2 && c->prm.cutoff <= 1e4;
+    a.lj = lj && !energy && !lam;
+    a.sig2 = c->sig_u * c->sig_u;
+    a.eps24 = 24.f * c->eps_u;
     a.nu = (float)c->prm.nu;
     a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
     a.kick = (!energy && c->kick_in == KICK_IN_ATM) ? 0.5 * c->prm.delta_t : 0.0;

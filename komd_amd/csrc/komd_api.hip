@@ -206,8 +206,12 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     const int atm = atm_code(kind3b);
     const int nl = nl_mode(kind3b);
     if (nl) KOMD_CHECK_HIP(ensure_nl(c, nl == 2));
-    KOMD_CHECK_HIP(launch_cell(c, true, nl, false, true));
-    if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr);
+    // uniform single-site LJ without the clamp: the three-body kernel sums the LJ force
+    // from the site's compacted list (one pair per lane) instead of the pair pass's
+    // candidate loop, where a lane's rare hits left most lanes idle
+    const bool lj_atm = atm && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+    KOMD_CHECK_HIP(launch_cell(c, !lj_atm, nl, false, true));
+    if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr, lj_atm);
     if (kind3b == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, false, 0);
     if (kind3b == KOMD_3B_HYBRID) return hybrid_forces(c, false);
     return hipSuccess;

@@ -186,7 +186,10 @@ hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins);
 // per-site energy share out (optional).
 enum { KICK_NONE = 0, KICK_IN_CELL = 1, KICK_IN_ATM = 2, KICK_IN_APPLY = 3 };
 hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit);
-hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite);
+// lj: the kernel also adds the LJ force of every listed pair (uniform single-site
+// systems without the force clamp; the pair pass then skips LJ)
+hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite,
+                      bool lj = false);
 int cell_blocks(komd_ctx* c);
 int atm_blocks(komd_ctx* c);
 // k_bonded.hip

@@ -182,6 +182,33 @@ def test_atm_exact_forces(orc, cells, seed):
     assert abs(obs["u_3b"] - e_or) < 1e-5 * abs(e_or)
 
 
+@pytest.mark.parametrize("cells,seed,rho", [(10, 45, 0.8), (5, 46, 1.1)])
+def test_step_forces_lj_in_three_body_kernel(orc, cells, seed, rho):
+    """The NVE step's force pass of a uniform single-site system without the clamp sums
+    LJ in the three-body kernel from each site's list (rotation path at density 0.8,
+    lists over 64 at 1.1): the total must be the oracle's LJ (C08 walk) plus ATM, and
+    equal the separate pair and three-body passes to rounding."""
+    s = S.fcc(cells, rho, 0.8, seed, jitter=0.1)
+    print(f"step forces n={s.n} rho={rho}", flush=True)
+    r, b = orc.update_container(s)
+    f_lj, _, _ = orc.lj_c08(s, r, b, energy=True, nthreads=8)
+    f_atm, _, _ = orc.atm_complete(s, r, b, NU, coverage=0, sign=-1.0, nthreads=8)
+    with K.Context(s, nu=NU) as c, K.Context(s, nu=NU) as d:
+        c.update_container()
+        c.compute_forces(K.ATM_EXACT)
+        f = c.forces()
+        d.update_container()
+        d.forces_pair()
+        d.forces_3b(K.ATM_EXACT)
+        f_sep = d.forces()
+    mx, rms = rel_err(f, f_lj + f_atm)
+    mx2, rms2 = rel_err(f, f_sep)
+    print(f"step forces vs oracle max {mx:.3e} rms {rms:.3e}; vs separate passes max {mx2:.3e} rms {rms2:.3e}")
+    assert mx < 2e-4 and rms < 5e-6
+    # the same FP32 pair terms, summed in FP64 in another order (r^2 may differ by an ulp)
+    assert mx2 < 1e-5 and rms2 < 1e-6
+
+
 def test_atm_exact_long_lists(orc):
     """Density 1.1: an FCC site has 78 neighbours within 2.5 (shells of 12, 6, 24, 12
     and 24), so every list is longer than 64 and takes the circulant LDS filter."""
