@@ -108,6 +108,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // LDS writes of this wave visible to this wave's later reads.
+// the lanes of the wave whose predicate holds (the builtin takes the predicate as a
+// lane mask; the int-taking __ballot rebuilds it from a VGPR)
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -119,7 +123,7 @@ template <bool UNIFORM, bool MOLS, bool LJ, int NL, bool ENERGY, bool LIMIT>
 __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     constexpr int kStage = kChunkLJ;
     constexpr int kWaves = kCellThreads / 64;
-    __shared__ float4 pos[kStage];
+    __shared__ float4 pos[kStage + 32];  // + 32: a group's lanes past the chunk read in bounds
     __shared__ uint16_t sdir[NL == 2 ? kStage : 1];
     __shared__ float2 cprm[UNIFORM ? 1 : kStage];
     __shared__ uint32_t cmol[MOLS ? kStage : 1];
@@ -186,8 +190,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     const int h = tid / lps, sub = tid % lps;
     // the lps lanes of one home site are consecutive lanes of one wave
     const int gshift = (lane / lps) * lps;
-    const unsigned long long gmask = (1ull << lps) - 1ull;
-    const unsigned long long below = (1ull << (lane - gshift)) - 1ull;
+    const unsigned long long gmask = ((1ull << lps) - 1ull) << gshift;  // the lanes of my home site
 
     for (int base = 0; live && base < nh; base += per) {
         const bool active = base + h < nh;
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
             for (int t0 = 0; t0 < mc; t0 += lps) {
                 const int t = t0 + sub;
                 const bool cand = active && t < mc;
-                const float4 c = pos[cand ? t : 0];
+                const float4 c = pos[t];  // t < kStage + 32; lanes without a candidate ignore it
                 const uint32_t w = __float_as_uint(c.w);
                 const uint32_t oj = w & kIdxMask;
                 const float dx = c.x - qi.x, dy = c.y - qi.y, dz = c.z - qi.z;
@@ -234,13 +237,14 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 bool in = cand && r2 <= a.rc2f;
                 // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
                 const bool amb = cand && fabsf(r2 - a.rc2f) <= a.band;
-                if (__ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
+                if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
                 const bool hit = in && (oj != oi);
-                npairs += (unsigned long long)__popcll(__ballot(hit && oj > oi));  // wave-uniform
+                npairs += (unsigned long long)__popcll(ballot(hit && oj > oi));  // wave-uniform
                 if (NL) {
-                    const unsigned long long bal = (__ballot(hit) >> gshift) & gmask;
+                    const unsigned long long bal = ballot(hit) & gmask;
                     if (hit) {
-                        const uint32_t s = ncnt + (uint32_t)__popcll(bal & below);
+                        const uint32_t s = ncnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
                         if (s < (uint32_t)kFullCap) {
                             const size_t slot = (size_t)p * kFullCap + s;
                             a.nl[slot] = make_float4(dx, dy, dz, __uint_as_float(w));
@@ -416,7 +420,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
         }
         wave_sync();
         // hybrid: a site whose neighbourhood lies wholly outside the exact region has no weighted triplet
-        if (HYB && __ballot(weighted) == 0ull) p_skip = true;
+        if (HYB && ballot(weighted) == 0ull) p_skip = true;
         if (!p_skip) {
             uint32_t* ring = qbuf[wvu];
             // the visit energy is needed by the energy passes, the hybrid weight gradient
@@ -441,9 +445,9 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
                 bool in = live && Z <= a.rc2f;
                 const float p2 = xv.x * zx + xv.y * zy + xv.z * zz;
                 const bool amb = live && fabsf(Z - a.rc2f) <= a.band3;
-                if (__ballot(amb) != 0ull && amb)
+                if (ballot(amb) != 0ull && amb)
                     in = !(exact_r2(a.rx, a.ry, a.rz, a.g, __float_as_uint(xv.w), __float_as_uint(yv.w)) > a.rc2);
-                nvisit += (unsigned long long)__popcll(__ballot(in));
+                nvisit += (unsigned long long)__popcll(ballot(in));
                 float wt = 1.f;
                 if (HYB) wt = (li + wlam[wvu][ia] + wlam[wvu][ib]) * (1.f / 3.f);
                 const float2 sx = wsq[wvu][ia], sy = wsq[wvu][ib];
@@ -476,7 +480,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
             int head = 0, tail = 0;
             // ballot-compact this instruction's candidates into the ring; dense flush at 64
             auto push = [&](bool hit, uint32_t e) {
-                const unsigned long long bal = __ballot(hit);
+                const unsigned long long bal = ballot(hit);
                 if (hit) {
                     const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
