@@ -102,16 +102,33 @@ __device__ __forceinline__ double exact_r2(const double* __restrict__ rx, const 
     return s;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+// v from the lane that DPP control `ctrl` selects (both halves of the double moved)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-// LDS writes of this wave visible to this wave's later reads.
+// Sum over the wave in a fixed order, the total in every lane: quads by quad_perm,
+// 16-lane rows by row_ror (DPP, no LDS), then the four row sums read out.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x124>(v);  // row_ror:4
+    v += dpp_f64<0x128>(v);  // row_ror:8
+    auto row = [&](int l) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                                __builtin_amdgcn_readlane(__double2loint(v), l));
+    };
+    return (row(0) + row(16)) + (row(32) + row(48));
+}
+
 // the lanes of the wave whose predicate holds (the builtin takes the predicate as a
 // lane mask; the int-taking __ballot rebuilds it from a VGPR)
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// LDS writes of this wave visible to this wave's later reads.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
