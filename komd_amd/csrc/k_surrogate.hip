@@ -294,11 +294,9 @@ __device__ __forceinline__ void desc_role(const DescArgs& a, int p, int sub, flo
         }
         desc_wave_sync();
     } else {
+        static_assert(kLps == 2 || kLps == 4 || kLps == 8 || kLps == 16, "DPP group sum");
 #pragma unroll
-        for (int o = 1; o < kLps; o <<= 1) {
-#pragma unroll
-            for (int k = 0; k < NACC; k++) acc[k] += __shfl_xor(acc[k], o);
-        }
+        for (int k = 0; k < NACC; k++) acc[k] = group_sum(acc[k], kLps);
     }
     if (p >= a.n) return;
     float4* xo = reinterpret_cast<float4*>(a.X + (size_t)p * ND);
@@ -807,11 +805,9 @@ __global__ __launch_bounds__(kSurForceThreads) void k_sur_force(ForceArgs a) {
                     }
                 }
             }
-            for (int o = 1; o < lps; o <<= 1) {
-                fx += __shfl_xor(fx, o);
-                fy += __shfl_xor(fy, o);
-                fz += __shfl_xor(fz, o);
-            }
+            fx = group_sum(fx, lps);  // lps = 16, 8 or 4
+            fy = group_sum(fy, lps);
+            fz = group_sum(fz, lps);
             if (live && sub == 0) {
                 if constexpr (!fixed) {
                     atomicAdd(&acc.f[0][self + h], fx);

@@ -19,6 +19,24 @@ __device__ __forceinline__ void sym_outer(T* w, T ax, T ay, T az, T bx, T by, T 
     w[5] += az * bz;
 }
 
+// v from the lane that DPP control CTRL selects within its 16-lane row
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over aligned groups of `lps` lanes (2, 4, 8 or 16; uniform), the group total in every
+// lane of the group: quad_perm [1,0,3,2] and [2,3,0,1], then row_half_mirror (lane i and
+// 7 - i of each 8) and row_mirror (i and 15 - i of each 16).  No LDS; a + b = b + a, so the
+// lanes of a group agree bit for bit.
+__device__ __forceinline__ float group_sum(float v, int lps) {
+    v += dpp_f32<0xB1>(v);
+    if (lps > 2) v += dpp_f32<0x4E>(v);
+    if (lps > 4) v += dpp_f32<0x141>(v);
+    if (lps > 8) v += dpp_f32<0x140>(v);
+    return v;
+}
+
 // Sum of v over the workgroup in a fixed order (blockDim.x a multiple of 64, at most
 // 1024); every thread returns the total.
 __device__ __forceinline__ double block_sum(double v) {
