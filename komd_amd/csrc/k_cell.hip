@@ -102,14 +102,6 @@ __device__ __forceinline__ double exact_r2(const double* __restrict__ rx, const 
     return s;
 }
 
-// v from the lane that DPP control `ctrl` selects (both halves of the double moved)
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
 // Sum over the wave in a fixed order, the total in every lane: quads by quad_perm,
 // 16-lane rows by row_ror (DPP, no LDS), then the four row sums read out.
 __device__ __forceinline__ double wave_sum(double v) {

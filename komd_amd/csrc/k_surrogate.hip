@@ -197,6 +197,13 @@ struct DescArgs {
     const uint32_t* __restrict__ sorted;
     int n;
     float pirc;
+    // LJ 12-6 of every listed pair added to the site's force by the radial role (the
+    // step's pair pass then only builds lists): uniform sites, no clamp, no hybrid
+    double* __restrict__ fx;
+    double* __restrict__ fy;
+    double* __restrict__ fz;
+    int lj;
+    float sig2, eps24;
 };
 
 // Radial window: the kWin Gaussians around y.  Every Gaussian outside
@@ -338,6 +345,32 @@ __device__ __forceinline__ void desc_role(const DescArgs& a, int p, int sub, flo
 #pragma unroll
         for (int k = 0; k < NR / 4; k++)
             if (k % kLps == sub) xo[k] = make_float4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]);
+        if (a.lj) {
+            // LJ 12-6 of the site's pairs, a second walk over its list (L2-resident) once the
+            // radial sums are stored: the pair pass's expression, FP32 per pair, force on
+            // i = -x fac, FP64 sums, combined over the quad in a fixed order
+            double ljx = 0.0, ljy = 0.0, ljz = 0.0;
+            for (int s = sub; s < nn; s += kLps) {
+                const float4 v = row[s];
+                const float inv = __builtin_amdgcn_rcpf(v.x * v.x + v.y * v.y + v.z * v.z);
+                const float s2 = a.sig2 * inv;
+                const float s6 = s2 * s2 * s2;
+                const float fac = a.eps24 * (2.f * s6 * s6 - s6) * inv;
+                ljx -= (double)(v.x * fac);
+                ljy -= (double)(v.y * fac);
+                ljz -= (double)(v.z * fac);
+            }
+            static_assert(kLps == 4, "LJ sum over a quad");
+            ljx += dpp_f64<0xB1>(ljx); ljx += dpp_f64<0x4E>(ljx);
+            ljy += dpp_f64<0xB1>(ljy); ljy += dpp_f64<0x4E>(ljy);
+            ljz += dpp_f64<0xB1>(ljz); ljz += dpp_f64<0x4E>(ljz);
+            if (sub == 0) {
+                const uint32_t oi = a.sorted[p];
+                a.fx[oi] = a.fx[oi] + ljx;
+                a.fy[oi] = a.fy[oi] + ljy;
+                a.fz[oi] = a.fz[oi] + ljz;
+            }
+        }
     }
 }
 
@@ -1397,6 +1430,10 @@ hipError_t launch_desc(komd_ctx* c, bool raw, const float* lam = nullptr) {
     ScopedTimer tm(c, "sur_desc");
     Surrogate* s = c->sur;
     DescArgs a{c->d.nl, c->d.nlc, raw ? nullptr : s->params, s->X, s->T, lam, c->d.sorted, (int)c->n, pirc_of(c)};
+    a.fx = c->d.fx; a.fy = c->d.fy; a.fz = c->d.fz;
+    a.lj = c->lj_in_desc && !raw && !lam;
+    a.sig2 = c->sig_u * c->sig_u;
+    a.eps24 = 24.f * c->eps_u;
     k_sur_desc<<<(int)((c->n + kDescSites - 1) / kDescSites), 256, 0, c->stream>>>(a);
     return hipGetLastError();
 }

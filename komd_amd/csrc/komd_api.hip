@@ -209,10 +209,17 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     // uniform single-site LJ without the clamp: the three-body kernel sums the LJ force
     // from the site's compacted list (one pair per lane) instead of the pair pass's
     // candidate loop, where a lane's rare hits left most lanes idle
-    const bool lj_atm = atm && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
-    KOMD_CHECK_HIP(launch_cell(c, !lj_atm, nl, false, true));
+    // (the surrogate's descriptor pass likewise walks every site's list)
+    const bool lj_mv = c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+    const bool lj_atm = atm && lj_mv, lj_sur = kind3b == KOMD_3B_SURROGATE && lj_mv;
+    KOMD_CHECK_HIP(launch_cell(c, !(lj_atm || lj_sur), nl, false, true));
     if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr, lj_atm);
-    if (kind3b == KOMD_3B_SURROGATE) return launch_surrogate_forces(c, false, 0);
+    if (kind3b == KOMD_3B_SURROGATE) {
+        c->lj_in_desc = lj_sur;
+        const hipError_t e = launch_surrogate_forces(c, false, 0);
+        c->lj_in_desc = false;
+        return e;
+    }
     if (kind3b == KOMD_3B_HYBRID) return hybrid_forces(c, false);
     return hipSuccess;
 }

@@ -125,6 +125,7 @@ struct komd_ctx {
     // Integrator::Step1 fused into the last force kernel of an NVE step (no kinetic sums
     // are needed there): 0 none (k_kick runs), else the kernel that applies it
     int kick_in = 0;
+    bool lj_in_desc = false;  // set by the NVE step's force pass: the descriptor pass also sums LJ
     int hyb_axis = 0;
     double hyb_center = 0, hyb_half = 0, hyb_blend = 0;
     // exact-ATM judge of surrogate runs

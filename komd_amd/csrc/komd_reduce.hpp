@@ -25,6 +25,14 @@ __device__ __forceinline__ float dpp_f32(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
+// the same for a double (both halves moved)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Sum over aligned groups of `lps` lanes (2, 4, 8 or 16; uniform), the group total in every
 // lane of the group: quad_perm [1,0,3,2] and [2,3,0,1], then row_half_mirror (lane i and
 // 7 - i of each 8) and row_mirror (i and 15 - i of each 16).  No LDS; a + b = b + a, so the

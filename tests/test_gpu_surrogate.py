@@ -107,6 +107,27 @@ def test_forces_bitwise_repeatable():
     assert np.array_equal(F[0], F[1])
 
 
+def test_step_forces_lj_in_descriptor_pass():
+    """The surrogate NVE step of a uniform single-site system sums LJ in the descriptor
+    pass (its pair pass only builds lists).  Per-pair FP32 terms sum exactly in FP64, so
+    the step's forces equal the separate pair and surrogate passes bit for bit; both of
+    those are pinned to their oracles elsewhere."""
+    s = system(8, 17)
+    p = params_for(s, 7)
+    with K.Context(s) as c, K.Context(s) as d:
+        c.surrogate_set_params(p)
+        c.compute_forces(K.SURROGATE)
+        f = c.forces()
+        d.surrogate_set_params(p)
+        d.update_container()
+        d.forces_pair()
+        d.forces_3b(K.SURROGATE)
+        f_sep = d.forces()
+    print(f"max |F| {np.abs(f).max():.3e}, max diff {np.abs(f - f_sep).max():.3e}", flush=True)
+    assert np.abs(f).max() > 0.1
+    assert np.array_equal(f, f_sep)
+
+
 def test_forces_large_scale_float_path():
     """Final-layer weights scaled by 1e4 scale every force by 1e4, beyond the fixed-point
     accumulator's per-add range: the blocks fall back to float adds and the result still
