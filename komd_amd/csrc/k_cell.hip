@@ -242,13 +242,16 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 const uint32_t w = __float_as_uint(c.w);
                 const uint32_t oj = w & kIdxMask;
                 const float dx = c.x - qi.x, dy = c.y - qi.y, dz = c.z - qi.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                bool in = cand && r2 <= a.rc2f;
+                // lanes without a candidate test +inf: every predicate below is then one
+                // compare, which the ballots take as a lane mask directly
+                const float r2 = cand ? dx * dx + dy * dy + dz * dz : __builtin_huge_valf();
+                bool in = r2 <= a.rc2f;
                 // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
-                const bool amb = cand && fabsf(r2 - a.rc2f) <= a.band;
+                const bool amb = fabsf(r2 - a.rc2f) <= a.band;
                 if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
-                const bool hit = in && (oj != oi);
-                npairs += (unsigned long long)__popcll(ballot(hit && oj > oi));  // wave-uniform
+                const uint32_t oh = in ? oj : oi;  // hit: in range and not i itself
+                const bool hit = oh != oi;
+                npairs += (unsigned long long)__popcll(ballot(oh > oi));  // wave-uniform
                 if (NL) {
                     const unsigned long long bal = ballot(hit) & gmask;
                     if (hit) {
