@@ -379,7 +379,7 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 // ATM: 1 physical, 2 reference semantics.
 template <int ATM, bool ENERGY, bool HYB>
-__global__ __launch_bounds__(64 * kAtmWaves) void k_atm(AtmArgs a) {
+__global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_atm(AtmArgs a) {
     __shared__ float4 wbuf[kAtmWaves][kFullCap];
     __shared__ float2 wsq[kAtmWaves][kFullCap];  // |x_a|^2 and its reciprocal per neighbour
     __shared__ uint32_t qbuf[kAtmWaves][128];  // ring of 128 candidate pairs
