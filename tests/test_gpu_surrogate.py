@@ -109,9 +109,11 @@ def test_forces_bitwise_repeatable():
 
 def test_step_forces_lj_in_descriptor_pass():
     """The surrogate NVE step of a uniform single-site system sums LJ in the descriptor
-    pass (its pair pass only builds lists).  Per-pair FP32 terms sum exactly in FP64, so
-    the step's forces equal the separate pair and surrogate passes bit for bit; both of
-    those are pinned to their oracles elsewhere."""
+    pass (its pair pass only builds lists).  Its forces equal the separate pair and
+    surrogate passes (both pinned to their oracles elsewhere) to rounding: the LJ terms
+    are summed in FP64 in another order, and with these parameters some per-pair
+    surrogate terms exceed the fixed-point range, so blocks sum in float in any order.
+    A missing or doubled LJ term would differ by the LJ force itself."""
     s = system(8, 17)
     p = params_for(s, 7)
     with K.Context(s) as c, K.Context(s) as d:
@@ -123,9 +125,10 @@ def test_step_forces_lj_in_descriptor_pass():
         d.forces_pair()
         d.forces_3b(K.SURROGATE)
         f_sep = d.forces()
-    print(f"max |F| {np.abs(f).max():.3e}, max diff {np.abs(f - f_sep).max():.3e}", flush=True)
-    assert np.abs(f).max() > 0.1
-    assert np.array_equal(f, f_sep)
+    scale = np.abs(f_sep).max()
+    print(f"max |F| {scale:.3e}, max diff {np.abs(f - f_sep).max():.3e}", flush=True)
+    assert scale > 0.1
+    assert np.abs(f - f_sep).max() <= 1e-6 * scale
 
 
 def test_forces_large_scale_float_path():
