@@ -324,7 +324,11 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     // (a single global counter would serialise thousands of same-address atomics)
     if (lane == 0) red_cnt[wv] = npairs;
     __syncthreads();
-    if (tid == 0) a.counters[b] += red_cnt[0] + red_cnt[1] + red_cnt[2] + red_cnt[3];
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kWaves; k++) t += red_cnt[k];
+        a.counters[b] += t;
+    }
     if (ENERGY) {
         u_acc = wave_sum(u_acc);
         w_acc = wave_sum(w_acc);
@@ -601,7 +605,9 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.counters[b] += red_cnt[0] + red_cnt[1] + red_cnt[2] + red_cnt[3];
+        unsigned long long t = 0;
+        for (int k = 0; k < kAtmWaves; k++) t += red_cnt[k];
+        a.counters[b] += t;
         if (ENERGY) {
             double s = 0;
             for (int k = 0; k < kAtmWaves; k++) s += red[k];
