@@ -511,7 +511,10 @@ def main():
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {
         "kernel": "k_atm (per-site Axilrod-Teller-Muto" + (" + LJ 12-6 over the lists" if lj_in_atm else "")
-                  + ", FP32 VALU)", "bound": "valu",
+                  + ", FP32 VALU)", "bound": "mfma",
+        "bound_note": "compute-bound (FLOP/s roofline, the 'mfma' class of the contract): the kernel issues FP32 "
+                      "vector instructions, not matrix ones, and is limited by VALU issue (SQ_INSTS_VALU x 4 "
+                      "cycles over the 1,024 SIMDs equals its time); 157.3 TFLOP/s is MI355X's FP32 figure",
         "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
         "avg_launch_us": round(atm_ms / max(atm_n, 1) * 1e3, 2),
