@@ -559,7 +559,8 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "atom-timesteps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (pair/triplet arithmetic), f64 (positions, force accumulation, reductions)",
+        "dtype": "f32",
+        "dtype_note": "f32 pair and triplet arithmetic; f64 positions, velocities, force accumulation and reductions",
         "data": "synthetic: seeded FCC argon lattice, Maxwell-Boltzmann velocities",
         "config": {"workload": f"exact 3-body argon, {n:,} atoms (FCC {cells} x 4), rho 0.8, T* 0.8, LJ 12-6 "
                                f"rc 2.5 + ATM nu 0.073 over every triplet within rc, dt 0.005, NVE, 1 GPU",
