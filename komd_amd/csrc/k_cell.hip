@@ -74,6 +74,10 @@ struct CellArgs {
     const double* __restrict__ mass;
     Grid g;
     int chunk, nblocks, split;
+    // cx > 0: XCD x owns cells [x cx, (x+1) cx); when its n cells fill one round of `slots`
+    // resident workgroups and part of a second, those past 2 slots - n are served by two
+    // workgroups each (placed last), so the second round holds half-cells only
+    int cx, slots;
     float rc2f, band, sig2, eps24, eps4;
     double rc2, limit, kick;
 };
@@ -146,8 +150,28 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int b = blockIdx.x;
-    const int lb = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: each XCD gets a contiguous slab
-    const int cell = lb / a.split, part = lb - cell * a.split;  // part: share of the cell's home sites
+    // XCD-aware: consecutive blocks go round-robin to the 8 XCDs; each XCD gets a contiguous slab
+    int cell, part, nparts;  // part of nparts: this workgroup's share of the cell's home sites
+    if (a.cx == 0) {
+        const int lb = (b & 7) * a.chunk + (b >> 3);
+        cell = lb / a.split;
+        part = lb - cell * a.split;
+        nparts = a.split;
+    } else {
+        const int x = b & 7, j = b >> 3;
+        const int n = max(0, min(a.cx, a.g.ncell - x * a.cx));
+        const int w = (n <= a.slots || n >= 2 * a.slots) ? n : 2 * a.slots - n;  // whole cells of this XCD
+        if (j < w) {
+            cell = x * a.cx + j;
+            part = 0;
+            nparts = 1;
+        } else {
+            const int i = j - w;
+            cell = w + (i >> 1) < n ? x * a.cx + w + (i >> 1) : a.g.ncell;  // past the slab: no cell
+            part = i & 1;
+            nparts = 2;
+        }
+    }
     double u_acc = 0.0, w_acc = 0.0;
     double vir[kSym] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // pair virial r_ij (x) F_ij (energy mode)
     unsigned long long npairs = 0;
@@ -189,7 +213,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
         hs = __builtin_amdgcn_readfirstlane(nb_start[13]);
         nh = __builtin_amdgcn_readfirstlane(nb_pref[14] - nb_pref[13]);
         // this workgroup's share of the home sites
-        const int nhp = (nh + a.split - 1) / a.split;
+        const int nhp = (nh + nparts - 1) / nparts;
         const int hb = min(nh, part * nhp);
         hs += hb;
         nh = min(nh - hb, nhp);
@@ -647,7 +671,32 @@ hipError_t dispatch(komd_ctx* c, const CellArgs& a, bool lj, int nl, bool energy
 
 }  // namespace
 
-int cell_blocks(komd_ctx* c) { return 8 * ((c->grid.ncell * c->cell_split + 7) / 8); }
+// Workgroups of the pair pass per XCD (blocks = 8 x that).  cx = 0: the plain mapping, with
+// `cell_split` workgroups per cell on small grids.  Otherwise XCD x owns cx consecutive cells;
+// with S < n < 2S cells for S resident slots, 2S - n of them get one workgroup and the rest
+// two (a partial second round of half-cells instead of whole ones); otherwise one each.
+static void cell_jobs(const komd_ctx* c, int* cx, int* chunk) {
+    const int C = c->grid.ncell;
+    if (c->cell_split > 1 || c->xcd_slots <= 0) {
+        *cx = 0;
+        *chunk = (C * c->cell_split + 7) / 8;
+        return;
+    }
+    *cx = (C + 7) / 8;
+    int jmax = 0;
+    for (int x = 0; x < 8; x++) {
+        const int n = std::max(0, std::min(*cx, C - x * *cx));
+        const int w = (n <= c->xcd_slots || n >= 2 * c->xcd_slots) ? n : 2 * c->xcd_slots - n;
+        jmax = std::max(jmax, w + 2 * (n - w));
+    }
+    *chunk = jmax;
+}
+
+int cell_blocks(komd_ctx* c) {
+    int cx, chunk;
+    cell_jobs(c, &cx, &chunk);
+    return 8 * chunk;
+}
 int atm_blocks(komd_ctx* c) { return 8 * (((int)((c->n + kAtmWaves - 1) / kAtmWaves) + 7) / 8); }
 
 hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
@@ -660,7 +709,8 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
     a.part = d.red; a.vpart = d.vpart; a.flags = d.flags; a.counters = d.cnt_pair;
     a.g = c->grid;
     a.split = c->cell_split;
-    a.chunk = (c->grid.ncell * a.split + 7) / 8;
+    cell_jobs(c, &a.cx, &a.chunk);
+    a.slots = c->xcd_slots;
     a.nblocks = 8 * a.chunk;
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;

@@ -373,6 +373,13 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
         const int64_t C0 = c->grid.ncell;
         const int64_t mean = (n + C0 - 1) / std::max<int64_t>(C0, 1);
         c->cell_split = C0 >= 1024 ? 1 : (int)std::min<int64_t>(8, std::max<int64_t>(1, (mean + 4) / 5));
+        if (c->xcd_slots == 0) {  // 8 XCDs; kCellWgPerCu workgroups of the pair pass per CU
+            int dev = 0;
+            hipDeviceProp_t p;
+            c->xcd_slots = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+                               ? std::max(1, p.multiProcessorCount / 8) * kCellWgPerCu
+                               : -1;
+        }
     }
     const int pair_blocks = cell_blocks(c);
     c->red_blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);

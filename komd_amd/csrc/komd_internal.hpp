@@ -110,6 +110,7 @@ struct komd_ctx {
     bool nl_valid = false;
     int red_blocks = 0;
     int cell_split = 1;        // workgroups sharing one home cell in the pair pass (small grids)
+    int xcd_slots = 0;         // pair pass: workgroups resident at once per XCD (0: not queried yet)
     // timing
     bool timing = false;
     std::unordered_map<std::string, komd::Timer> timers;
