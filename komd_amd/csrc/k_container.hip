@@ -13,6 +13,9 @@ namespace komd {
 namespace {
 
 constexpr int kBlock = 256;
+// k_com_wrap_key: one molecule per thread and no shared memory; 64-thread workgroups spread a
+// small system's few waves over all CUs (32k sites: 500 workgroups instead of 125)
+constexpr int kKeyBlock = 64;
 
 __device__ __forceinline__ int cell_coord(double com, double low, double cs, int nd) {
     // LinkedCells.cpp:93-94: fBin = (com - low)/cell_size; clamp((int64)fBin, 0, n-1)
@@ -375,7 +378,7 @@ hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift) {
     const bool site = c->prm.bin_mode == KOMD_BIN_SITE;
     const int units = site ? (int)c->n : (int)c->nmol;
     const uint32_t* ms = (site || !c->multisite) ? nullptr : d.mol_start;
-    k_com_wrap_key<<<(units + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+    k_com_wrap_key<<<(units + kKeyBlock - 1) / kKeyBlock, kKeyBlock, 0, c->stream>>>(
         units, ms, d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.key, d.slot, d.count, c->grid,
         site ? 1 : 0, do_drift ? 1 : 0, c->prm.delta_t, c->prm.delta_t * 0.5, nullptr, d.flags);
     return hipGetLastError();
@@ -388,7 +391,7 @@ hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev) {
     const bool site = c->prm.bin_mode == KOMD_BIN_SITE;
     const int units = site ? (int)c->n : (int)c->nmol;
     const uint32_t* ms = (site || !c->multisite) ? nullptr : d.mol_start;
-    k_com_wrap_key<<<(units + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+    k_com_wrap_key<<<(units + kKeyBlock - 1) / kKeyBlock, kKeyBlock, 0, c->stream>>>(
         units, ms, d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.key, d.slot, d.count, c->grid,
         site ? 1 : 0, 1, c->prm.delta_t, c->prm.delta_t * 0.5, beta_dev, d.flags);
     return hipGetLastError();
