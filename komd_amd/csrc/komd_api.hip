@@ -116,7 +116,8 @@ komd_status sync_check(komd_ctx* c, const char* where) {
         hipMemcpy(c->d.flags, &zero, sizeof(uint32_t), hipMemcpyHostToDevice);
         if (flags & FLAG_NL_OVERFLOW)
             return fail(c, KOMD_ERR_OVERFLOW,
-                        std::string(where) + ": a site has more than 64 in-range neighbours with a higher index");
+                        std::string(where) + ": a site has more than " + std::to_string(kFullCap) +
+                            " in-range neighbours (the neighbour-list capacity)");
         if (flags & FLAG_CELL_OVERFLOW)
             return fail(c, KOMD_ERR_OVERFLOW,
                         std::string(where) + ": a cell neighbourhood holds more than 768 sites or a cell more "

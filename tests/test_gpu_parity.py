@@ -228,6 +228,22 @@ def test_atm_exact_long_lists(orc):
     assert mx < 1e-4 and rms < 1e-5
 
 
+def test_neighbour_list_overflow_is_reported(orc):
+    """Density 1.7: an FCC site has 134 neighbours within 2.5 (shells of 12, 6, 24, 12,
+    24, 8 and 48; 111 to 128 with this jitter), more than the 96-entry list.  The pair pass stores the first 96 only
+    and raises the overflow flag, which comes back as KOMD_ERR_OVERFLOW; the next
+    in-capacity pass on the same context succeeds (the flag is cleared)."""
+    s = S.fcc(6, 1.7, 0.8, 45, jitter=0.02)  # box 7.98: a 3 x 3 x 3 grid
+    print(f"list overflow n={s.n}", flush=True)
+    with K.Context(s, nu=NU) as c:
+        c.update_container()
+        with pytest.raises(K.KomdError, match="neighbour-list capacity"):
+            c.forces_3b(K.ATM_EXACT)
+            c.forces()
+        c.forces_pair()
+        c.forces()
+
+
 @pytest.mark.parametrize("cells,seed", [(5, 51), (10, 52)])
 def test_atm_reference_mode(orc, cells, seed):
     """Reference ATM_NOLIST semantics: +grad V on the C08 triplet set."""
