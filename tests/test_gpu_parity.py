@@ -244,6 +244,25 @@ def test_neighbour_list_overflow_is_reported(orc):
         c.forces()
 
 
+def test_nonfinite_position_is_reported():
+    """A NaN or infinite coordinate is binned into an edge cell (LinkedCells.cpp:93-94's
+    clamp; NaN counts as below the grid) and flagged by the container update, which
+    fails with KOMD_ERR_NONFINITE; the context then takes finite positions again."""
+    s = S.fcc(5, 0.8, 0.8, 46, jitter=0.05)
+    print(f"nonfinite n={s.n}", flush=True)
+    for bad in (np.nan, np.inf):
+        t = s.copy()
+        t.r = t.r.copy()
+        t.r[17, 1] = bad
+        with K.Context(t) as c:
+            with pytest.raises(K.KomdError, match="non-finite"):
+                c.update_container()
+            c.set_sites(s)
+            c.update_container()
+            c.forces_pair()
+            assert np.isfinite(c.forces()).all()
+
+
 @pytest.mark.parametrize("cells,seed", [(5, 51), (10, 52)])
 def test_atm_reference_mode(orc, cells, seed):
     """Reference ATM_NOLIST semantics: +grad V on the C08 triplet set."""
