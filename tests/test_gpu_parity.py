@@ -35,9 +35,10 @@ def small(seed=11, cells=5, jitter=0.12, T=0.8):
     return S.fcc(cells, 0.8, T, seed, jitter=jitter)
 
 
-@pytest.mark.parametrize("cells,seed", [(5, 11), (10, 12), (20, 13), (30, 14)])
+@pytest.mark.parametrize("cells,seed", [(5, 11), (10, 12), (20, 13), (30, 14), (40, 15)])
 def test_binning_bit_exact(orc, cells, seed):
-    """cells 30: 20^3 = 8,000 domain cells, the tiled two-launch scan"""
+    """cells 30: 20^3 = 8,000 domain cells, the tiled two-launch scan; cells 40: the
+    256,000-atom workload (27^3 = 19,683 cells)"""
     s = small(seed, cells)
     # push some sites outside the box so the wrap path runs
     s.r[::97] += np.array([s.box[0] * 0.999, -0.3, 0.0])
