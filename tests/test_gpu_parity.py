@@ -163,8 +163,9 @@ def test_cutoff_decision_unfused(orc):
     assert npair == sum(want)
 
 
-@pytest.mark.parametrize("cells,seed", [(5, 41), (10, 42), (20, 43)])
+@pytest.mark.parametrize("cells,seed", [(5, 41), (10, 42), (20, 43), (40, 44)])
 def test_atm_exact_forces(orc, cells, seed):
+    """cells 20 and 40: the 32,000- and 256,000-atom workloads (6.3 M and 51 M triplets)"""
     s = small(seed, cells)
     print(f"atm exact cells={cells} n={s.n}", flush=True)
     r, b = orc.update_container(s)
