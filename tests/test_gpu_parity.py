@@ -105,8 +105,9 @@ def test_temperature_and_rescale(orc):
         np.testing.assert_allclose(v, s.v * np.sqrt(0.8 / T), rtol=1e-15, atol=0)
 
 
-@pytest.mark.parametrize("cells,seed", [(5, 31), (10, 32), (20, 33)])
+@pytest.mark.parametrize("cells,seed", [(5, 31), (10, 32), (20, 33), (40, 34)])
 def test_lj_forces(orc, cells, seed):
+    """cells 20 and 40: the 32,000- and 256,000-atom workloads"""
     s = small(seed, cells)
     print(f"lj cells={cells} n={s.n}", flush=True)
     r, b = orc.update_container(s)
