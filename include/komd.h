@@ -69,7 +69,8 @@ typedef struct {
     double nu;            /* ATM strength, caller units (ATM.cpp:13 converts before use)   */
     double temp_factor;   /* T = temp_factor * sum(m v.v)/(3 N_in_box) (TemperatureSensor)  */
     int32_t bin_mode;     /* KOMD_BIN_COM or KOMD_BIN_SITE                                 */
-    int32_t max_neighbors;/* half neighbour list capacity per site (0: 64, max 128)         */
+    int32_t max_neighbors;/* full neighbour-list capacity per site: fixed at 96; 0 or 96,     *
+                           * any other value fails komd_create with KOMD_ERR_ARG              */
     int32_t reserved[4];
 } komd_params;
 
@@ -116,7 +117,8 @@ komd_status komd_grid_dims(const komd_ctx* ctx, int64_t dims[3]);
  * r, v interleaved d3 (3n doubles), per-site mass/eps/sig, molecule id per site.
  * Consecutive sites with equal ids form one molecule (CheckpointIO.cpp:63-72).
  * Replaces MoleculeContainer::constructSOAs (MoleculeContainer.cpp:37-53).
- * Forces are zeroed. */
+ * Forces are zeroed.  A call with a different site count than the last one drops the
+ * surrogate training set (its snapshots hold the old count); surrogate parameters stay. */
 komd_status komd_set_sites(komd_ctx* ctx, int64_t n, const double* r, const double* v, const double* mass,
                            const double* eps, const double* sig, const uint64_t* mol_id);
 /* Overwrites positions / velocities / forces (any pointer may be NULL). */
@@ -146,7 +148,8 @@ komd_status komd_get_cells(komd_ctx* ctx, int64_t* site_cell, uint32_t* cell_cou
 
 /* LJ 12-6 over all site pairs of different molecules within cutoff whose cells are
  * neighbours (ForceFunctor::operator() -> LJ12_6::handlePairList, LJ12_6.cpp:10-43).
- * Adds to the forces.  Also builds the half neighbour list the 3-body kernels read. */
+ * Adds to the forces.  Builds no neighbour list: komd_forces_3b builds the full lists
+ * (every in-range neighbour of every site, capacity 96) that its kernels read. */
 komd_status komd_forces_pair(komd_ctx* ctx, int32_t kind);
 /* Harmonic bond and angle forces (new; replaces the reference's FENE functor slot). */
 komd_status komd_forces_bonded(komd_ctx* ctx);
@@ -181,7 +184,11 @@ komd_status komd_rdf(komd_ctx* ctx, double max_r, double delta_r, int64_t nbins,
  * {integrate0, update_container, LJ, [bonded], [limit], [3-body], integrate1,
  *  [temperature + rescale to t0 when t0 > 0]}.  kind3b: 0 none, else KOMD_3B_*.
  * use_bonded: run the bonded kernel.  Forces must be current on entry
- * (komd_compute_forces).  Synchronises once at the end. */
+ * (komd_compute_forces).  Synchronises once at the end.
+ * Kernel-raised errors (KOMD_ERR_OVERFLOW, KOMD_ERR_NONFINITE) are checked every 32
+ * steps without draining the queue, so up to 64 further steps may have run on capped
+ * lists or non-finite values before the call returns the error: after such an error the
+ * state is undefined (restore it with komd_set_state). */
 komd_status komd_run(komd_ctx* ctx, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0);
 /* The force part of one step: update_container + LJ + [bonded] + [limit] + [3-body]
  * (the pre-loop pass of Simulation.cpp:15-24). */

@@ -344,22 +344,36 @@ __device__ double sum_partials(const double* __restrict__ p, int nparts) {
 // T = temp_factor * mv2/(3 count); beta = sqrt(t0/T); v *= beta
 // (TemperatureSensor.cpp:41-42, VelocityScaling.cpp:12-25).  Every block sums
 // the kinetic partials itself in the same order, so beta is identical everywhere.
+// t0 comes from scal[kScalT0] when t0_dev is set (the captured step of komd_run, so
+// that a replay uses the target of the current run, not the one of the capture).
 __global__ __launch_bounds__(kBlock) void k_rescale(int n, double* __restrict__ vx, double* __restrict__ vy,
                                                     double* __restrict__ vz, const double* __restrict__ part,
-                                                    int nparts, double temp_factor, double t0,
+                                                    int nparts, double temp_factor, double t0, int t0_dev,
                                                     double* __restrict__ scal, int apply) {
     const double mv2 = sum_partials(part, nparts);
     const double cnt = sum_partials(part + nparts, nparts);
     const double ke = sum_partials(part + 2 * nparts, nparts);
     const double T = temp_factor * (mv2 / (cnt * 3.0));
-    const double beta = sqrt(t0 / T);
+    const double beta = sqrt((t0_dev ? scal[kScalT0] : t0) / T);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[0] = mv2; scal[1] = cnt; scal[2] = beta; scal[3] = T; scal[4] = 0.5 * ke;
+        scal[kScalMv2] = mv2; scal[kScalCount] = cnt; scal[kScalBeta] = beta; scal[kScalTemp] = T;
+        scal[kScalKin] = 0.5 * ke;
     }
     if (!apply) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         vx[i] = vx[i] * beta; vy[i] = vy[i] * beta; vz[i] = vz[i] * beta;
     }
+}
+
+__global__ void k_run_scalars(double* __restrict__ scal, double t0) {
+    if (threadIdx.x == 0) {
+        scal[kScalBeta] = 1.0;
+        scal[kScalT0] = t0;
+    }
+}
+
+__global__ void k_flag_snapshot(const uint32_t* __restrict__ flags, uint32_t* __restrict__ host_slot) {
+    if (threadIdx.x == 0) host_slot[0] = flags[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restrict__ part, int nparts, int nslots,
@@ -384,7 +398,7 @@ hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift) {
     return hipGetLastError();
 }
 
-// rescale-fused variant used by komd_run: v *= beta (from scal[2]) before the kick.
+// rescale-fused variant used by komd_run: v *= beta (from scal[kScalBeta]) before the kick.
 hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev) {
     ScopedTimer tm(c, "bin_key");
     auto& d = c->d;
@@ -458,8 +472,8 @@ hipError_t launch_temperature(komd_ctx* c) {
     k_kick<<<c->red_blocks, kBlock, 0, c->stream>>>(n, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.rx, d.ry, d.rz,
                                                      0.0, 1, c->grid, d.red, d.flags);
     KOMD_CHECK_HIP(hipGetLastError());
-    k_rescale<<<1, kBlock, 0, c->stream>>>(n, d.vx, d.vy, d.vz, d.red, c->red_blocks, c->prm.temp_factor, 1.0, d.scal,
-                                           0);
+    k_rescale<<<1, kBlock, 0, c->stream>>>(n, d.vx, d.vy, d.vz, d.red, c->red_blocks, c->prm.temp_factor, 1.0, 0,
+                                           d.scal, 0);
     return hipGetLastError();
 }
 
@@ -469,14 +483,24 @@ hipError_t launch_rescale(komd_ctx* c, double t0) {
     const int n = (int)c->n;
     const int blocks = std::min(1024, (n + kBlock - 1) / kBlock);
     k_rescale<<<blocks, kBlock, 0, c->stream>>>(n, d.vx, d.vy, d.vz, d.red, c->red_blocks, c->prm.temp_factor, t0,
-                                                d.scal, 1);
+                                                0, d.scal, 1);
     return hipGetLastError();
 }
 
 hipError_t launch_beta_only(komd_ctx* c) {
     auto& d = c->d;
     k_rescale<<<1, kBlock, 0, c->stream>>>((int)c->n, d.vx, d.vy, d.vz, d.red, c->red_blocks, c->prm.temp_factor,
-                                           c->run_t0, d.scal, 0);
+                                           0.0, 1, d.scal, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_run_scalars(komd_ctx* c, double t0) {
+    k_run_scalars<<<1, 64, 0, c->stream>>>(c->d.scal, t0);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_snapshot(komd_ctx* c, uint32_t* host_slot) {
+    k_flag_snapshot<<<1, 64, 0, c->stream>>>(c->d.flags, host_slot);
     return hipGetLastError();
 }
 

@@ -1397,15 +1397,26 @@ Surrogate* sur_of(komd_ctx* c) {
     return c->sur;
 }
 
+// Frees every buffer whose size follows the site count, and the training snapshots.
+void free_site_buffers(Surrogate* s) {
+    void* ptrs[] = {s->X, s->T, s->S, s->E, s->W, s->delta, s->fj, s->save, s->lam, s->dlam, s->lpart};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    s->X = s->T = s->S = s->E = s->W = s->fj = s->lam = s->dlam = nullptr;
+    s->delta = nullptr;
+    s->save = s->lpart = nullptr;
+    for (auto p : s->snap_r) hipFree(p);
+    for (auto p : s->snap_f) hipFree(p);
+    for (auto p : s->snap_e) hipFree(p);
+    s->snap_r.clear();
+    s->snap_f.clear();
+    s->snap_e.clear();
+}
+
 hipError_t ensure_site_buffers(komd_ctx* c) {
     Surrogate* s = sur_of(c);
     if (s->n != c->n) {
-        void* ptrs[] = {s->X, s->T, s->S, s->E, s->W, s->delta, s->fj, s->save, s->lam, s->dlam};
-        for (void* p : ptrs)
-            if (p) hipFree(p);
-        s->X = s->T = s->S = s->E = s->W = s->fj = s->lam = s->dlam = nullptr;
-        s->delta = nullptr;
-        s->save = nullptr;
+        free_site_buffers(s);
         s->n = c->n;
     }
     const size_t n = (size_t)c->n;
@@ -1560,6 +1571,14 @@ hipError_t surrogate_prepare(komd_ctx* c, bool hybrid) {
     return hipSuccess;
 }
 
+void surrogate_sites_changed(komd_ctx* c) {
+    Surrogate* s = c->sur;
+    if (!s || s->n == c->n) return;
+    hipStreamSynchronize(c->stream);
+    free_site_buffers(s);
+    s->n = c->n;
+}
+
 void surrogate_free(komd_ctx* c) {
     Surrogate* s = c->sur;
     if (!s) return;
@@ -1598,6 +1617,7 @@ komd_status need(komd_ctx* c) {
 // save / restore positions and forces around passes that rebuild the container
 hipError_t save_state(komd_ctx* c) {
     Surrogate* s = sur_of(c);
+    KOMD_CHECK_HIP(ensure_site_buffers(c));  // buffers of the current site count
     KOMD_CHECK_HIP(salloc(&s->save, 6 * (size_t)c->n));
     const size_t nb = (size_t)c->n * 8;
     double* src[6] = {c->d.rx, c->d.ry, c->d.rz, c->d.fx, c->d.fy, c->d.fz};
@@ -1778,6 +1798,8 @@ komd_status komd_train_add_snapshot(komd_ctx* c) {
     komd_status st = need(c);
     if (st) return st;
     Surrogate* s = sur_of(c);
+    // buffers of the current site count first: a change of count drops the stored set
+    SCALL(c, ensure_site_buffers(c), "komd_train_add_snapshot");
     const size_t n = (size_t)c->n;
     double* r = nullptr;
     float *f = nullptr, *e = nullptr;
@@ -1867,9 +1889,9 @@ komd_status komd_train_standardize(komd_ctx* c) {
         SCALL(c, launch_desc(c, true), "komd_train_standardize");
         k_sur_colstats<<<kJudgeBlocks, 128, 0, c->stream>>>((int)n, s->X, s->stats);
         SCALL(c, hipGetLastError(), "komd_train_standardize");
-        SCALL(c, hipMemcpyAsync(part.data(), s->stats, part.size() * 8, hipMemcpyDeviceToHost, c->stream),
-              "komd_train_standardize");
+        // host buffers are pageable: synchronous copies after the stream has drained
         SCALL(c, hipStreamSynchronize(c->stream), "komd_train_standardize");
+        SCALL(c, hipMemcpy(part.data(), s->stats, part.size() * 8, hipMemcpyDeviceToHost), "komd_train_standardize");
         for (int b = 0; b < kJudgeBlocks; b++)
             for (int j = 0; j < 2 * ND; j++) tot[(size_t)j] += part[(size_t)b * 2 * ND + j];
     }
@@ -1959,12 +1981,11 @@ komd_status komd_surrogate_judge(komd_ctx* c, double out[2]) {
     SCALL(c, launch_atm(c, 1, false, nullptr, nullptr, nullptr), "komd_surrogate_judge");
     k_sur_judge<<<kJudgeBlocks, 256, 0, c->stream>>>((int)c->n, s->fj, c->d.fx, c->d.fy, c->d.fz, s->stats);
     SCALL(c, hipGetLastError(), "komd_surrogate_judge");
-    std::vector<double> part(3 * kJudgeBlocks);
-    SCALL(c, hipMemcpyAsync(part.data(), s->stats, part.size() * 8, hipMemcpyDeviceToHost, c->stream),
-          "komd_surrogate_judge");
     SCALL(c, restore_state(c), "komd_surrogate_judge");
     st = api_sync(c, "komd_surrogate_judge");
     if (st) return st;
+    std::vector<double> part(3 * kJudgeBlocks);  // pageable: a synchronous copy after the drain
+    SCALL(c, hipMemcpy(part.data(), s->stats, part.size() * 8, hipMemcpyDeviceToHost), "komd_surrogate_judge");
     double se = 0, sr = 0, mx = 0;
     for (int b = 0; b < kJudgeBlocks; b++) {
         se += part[3 * b];

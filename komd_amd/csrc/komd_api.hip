@@ -289,6 +289,13 @@ komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx
         return KOMD_ERR_ARG;
     }
     g.ncell = g.nx * g.ny * g.nz;
+    if (params->max_neighbors != 0 && params->max_neighbors != kFullCap) {
+        c->err = "max_neighbors: the neighbour-list capacity is fixed at " + std::to_string(kFullCap) +
+                 " sites (pass 0 or " + std::to_string(kFullCap) + ")";
+        *out = c;
+        return KOMD_ERR_ARG;
+    }
+    c->prm.max_neighbors = kFullCap;
     if (c->prm.temp_factor <= 0) c->prm.temp_factor = 1.0;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = dalloc(&c->d.scal, 32);
@@ -297,6 +304,8 @@ komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx
     if (e == hipSuccess) e = hipMemset(c->d.flags, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->d.counters, 0, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->d.scal, 0, 32 * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->flag_host, 2 * sizeof(uint32_t));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&c->flag_ev[k], hipEventDisableTiming);
     *out = c;
     if (e != hipSuccess) return hip_fail(c, e, "komd_create");
     return KOMD_OK;
@@ -311,6 +320,9 @@ void komd_destroy(komd_ctx* c) {
     if (c->d.scal) hipFree(c->d.scal);
     if (c->d.flags) hipFree(c->d.flags);
     if (c->d.counters) hipFree(c->d.counters);
+    if (c->flag_host) hipHostFree(c->flag_host);
+    for (auto e : c->flag_ev)
+        if (e) hipEventDestroy(e);
     if (c->d.bonds) hipFree(c->d.bonds);
     if (c->d.angles) hipFree(c->d.angles);
     if (c->d.bond_mol) hipFree(c->d.bond_mol);
@@ -341,6 +353,7 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     drop_graph(c);
     free_sites(c);
     c->n = n;
+    surrogate_sites_changed(c);
     auto& d = c->d;
     // molecules: runs of equal consecutive ids (CheckpointIO.cpp:63-72); id classes for the exclusion
     std::vector<uint32_t> mstart;
@@ -658,12 +671,9 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
     komd_status s = KOMD_OK;
     const bool bonded = use_bonded && (c->nbonds + c->nangles) > 0;
     const bool thermo = t0 > 0;
-    if (thermo) {
-        const double one = 1.0;
-        HIPCALL(c, hipMemcpyAsync(c->d.scal + 2, &one, sizeof(double), hipMemcpyHostToDevice, c->stream), "komd_run");
-        HIPCALL(c, hipStreamSynchronize(c->stream), "komd_run");
-    }
-    c->run_t0 = t0;
+    // beta = 1 for the first kick-drift and the target the captured step reads, stream-ordered
+    // ahead of the steps (the graph holds no host value of t0, so one graph serves every target)
+    if (thermo) HIPCALL(c, launch_run_scalars(c, t0), "komd_run");
     {
         // device buffers of the chosen passes exist before any graph capture (no allocation while capturing)
         const int atm = atm_code(kind3b);
@@ -704,11 +714,27 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
             HIPCALL(c, e, "komd_run instantiate");
             c->graph_key = key;
         }
+        // Every kFlagEvery steps the error flags are copied to a pinned host slot; the host
+        // reads the previous copy (kFlagEvery steps older) when it queues the next one, so an
+        // overflow or a non-finite state stops the run within 2 kFlagEvery steps without
+        // draining the queue at every check.
+        constexpr int64_t kFlagEvery = 32;
+        bool pending[2] = {false, false};
         for (int64_t k = 0; k < nsteps; k++) {
             HIPCALL(c, hipGraphLaunch(c->graph_exec, c->stream), "komd_run launch");
-            if ((k % 256) == 255) {
-                s = sync_check(c, "komd_run");
-                if (s) return s;
+            if ((k + 1) % kFlagEvery == 0 && k + 1 < nsteps) {
+                const int slot = (int)(((k + 1) / kFlagEvery) & 1), prev = slot ^ 1;
+                if (pending[prev]) {
+                    HIPCALL(c, hipEventSynchronize(c->flag_ev[prev]), "komd_run");
+                    pending[prev] = false;
+                    if (c->flag_host[prev]) {
+                        s = sync_check(c, "komd_run");
+                        if (s) return s;
+                    }
+                }
+                HIPCALL(c, launch_flag_snapshot(c, c->flag_host + slot), "komd_run");
+                HIPCALL(c, hipEventRecord(c->flag_ev[slot], c->stream), "komd_run");
+                pending[slot] = true;
             }
         }
     }
@@ -779,23 +805,20 @@ komd_status komd_judge_report(komd_ctx* c, int64_t* checks, double* last_rel_rms
     return KOMD_OK;
 }
 
-komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* out) {
-    komd_status s = need_sites(c);
-    if (s) return s;
-    if (!out) return KOMD_ERR_ARG;
-    memset(out, 0, sizeof(*out));
+}  // extern "C"
+
+namespace {
+// The energy passes of komd_measure_observables; they overwrite the forces, which the
+// caller restores on every exit.
+komd_status measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* out, const std::vector<double>& fsave) {
+    komd_status s = KOMD_OK;
     auto& d = c->d;
-    const int64_t n = c->n;
-    // keep the current forces: the energy passes rebuild them
-    std::vector<double> fsave((size_t)(3 * n));
-    s = komd_get_sites(c, nullptr, nullptr, fsave.data());
-    if (s) return s;
     int64_t cnt0[2];
     s = read_counters(c, cnt0, false);
     if (s) return s;
     HIPCALL(c, launch_com_wrap_key(c, false), "komd_measure_observables");
     HIPCALL(c, launch_bin(c), "komd_measure_observables");
-    double* out_dev = d.scal + 24;  // [24, 32): results of one reduction
+    double* out_dev = d.scal + kScalOut;
     double vals[8];
     // reduce `nslots` slots of `nparts` partials and copy them to vals
     auto reduce = [&](const double* part, int nparts, int nslots) -> komd_status {
@@ -861,11 +884,33 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
     out->temperature = sc[3];
     out->kinetic = sc[4];
     out->n_in_box = (int64_t)sc[1];
-    // forces back as they were; fmax from them
-    double fm = 0;
+    double fm = 0;  // from the forces as they were
     for (double x : fsave) fm = std::max(fm, fabs(x));
     out->fmax = fm;
-    return komd_set_state(c, nullptr, nullptr, fsave.data());
+    return KOMD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* out) {
+    komd_status s = need_sites(c);
+    if (s) return s;
+    if (!out) return KOMD_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    // keep the current forces: the energy passes rebuild them
+    std::vector<double> fsave((size_t)(3 * c->n));
+    s = komd_get_sites(c, nullptr, nullptr, fsave.data());
+    if (s) return s;
+    s = measure_observables(c, kind3b, out, fsave);
+    const std::string first_err = c->err;
+    // forces back as they were on every exit, also after a failed pass
+    const komd_status s2 = komd_set_state(c, nullptr, nullptr, fsave.data());
+    if (s) {
+        c->err = first_err;
+        return s;
+    }
+    return s2;
 }
 
 komd_status komd_rdf(komd_ctx* c, double max_r, double delta_r, int64_t nbins, uint64_t* counts) {

@@ -33,6 +33,19 @@ constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
 // error flags raised by kernels (bitmask in dev->flags)
 enum : uint32_t { FLAG_NL_OVERFLOW = 1u, FLAG_NONFINITE = 2u, FLAG_CELL_OVERFLOW = 4u };
 
+// Slots of the device scalars (DevState::scal, 32 doubles).
+enum : int {
+    kScalMv2 = 0,    // sum m v.v of the sites inside the box (last temperature pass)
+    kScalCount = 1,  // sites inside the box
+    kScalBeta = 2,   // rescale factor applied at the next kick-drift of a thermostatted run
+    kScalTemp = 3,   // last measured temperature
+    kScalKin = 4,    // kinetic energy
+    kScalT0 = 5,     // thermostat target of the running komd_run (read by the captured step)
+    kScalHybU = 12,  // hybrid pass: exact-ATM energy share
+    kScalHybW = 16,  // hybrid pass: exact-ATM virial share [16, 22)
+    kScalOut = 24    // results of one reduction [24, 32)
+};
+
 struct Grid {
     int nx, ny, nz;         // domain cells
     int ncell;              // nx*ny*nz
@@ -121,8 +134,9 @@ struct komd_ctx {
     int graph_key = -1;
     // surrogate
     komd::Surrogate* sur = nullptr;
-    // hybrid
-    double run_t0 = 0;         // thermostat target of the running komd_run
+    // pinned host slots the flag snapshots of komd_run land in, and their events
+    uint32_t* flag_host = nullptr;
+    hipEvent_t flag_ev[2] = {nullptr, nullptr};
     // Integrator::Step1 fused into the last force kernel of an NVE step (no kinetic sums
     // are needed there): 0 none (k_kick runs), else the kernel that applies it
     int kick_in = 0;
@@ -176,7 +190,11 @@ __device__ __forceinline__ void kick_site(double* __restrict__ vx, double* __res
 }
 hipError_t launch_temperature(komd_ctx* c);     // kinetic partials -> scal[0..1]
 hipError_t launch_rescale(komd_ctx* c, double t0);    // beta from scal, v *= beta
-hipError_t launch_beta_only(komd_ctx* c);             // beta = sqrt(run_t0/T) into scal[2]
+hipError_t launch_beta_only(komd_ctx* c);             // beta = sqrt(scal[kScalT0]/T) into scal[kScalBeta]
+// scal[kScalBeta] = 1 and scal[kScalT0] = t0 before a thermostatted komd_run (outside any capture)
+hipError_t launch_run_scalars(komd_ctx* c, double t0);
+// copies the error flags to a pinned host slot (flag checks that do not drain the stream)
+hipError_t launch_flag_snapshot(komd_ctx* c, uint32_t* host_slot);
 hipError_t launch_reduce_parts(komd_ctx* c, const double* part, int nparts, int nslots, double* out_dev);
 hipError_t launch_kin_tensor(komd_ctx* c);      // sum m v_a v_b partials -> d.vpart ([6][kKinBlocks])
 constexpr int kKinBlocks = 256;
@@ -207,6 +225,9 @@ int surrogate_energy_blocks(komd_ctx* c);
 int surrogate_virial_blocks(komd_ctx* c);
 hipError_t surrogate_prepare(komd_ctx* c, bool hybrid);  // allocates the pass buffers (not capturable)
 void surrogate_free(komd_ctx* c);
+// komd_set_sites hook: when the site count changes, frees the surrogate's per-site buffers
+// and the training snapshots (both sized by the old count); parameters are kept
+void surrogate_sites_changed(komd_ctx* c);
 
 }  // namespace komd
 

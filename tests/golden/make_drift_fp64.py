@@ -1,12 +1,13 @@
 """FP64 energy trajectory of the exact three-body argon system from the oracle (CPU).
 
-System: 4,000 argon atoms (FCC 10^3 x 4, rho 0.8, T* 0.8, seed 1003), LJ 12-6 rc 2.5 +
+System: FCC argon (default 4,000 atoms, FCC 10^3 x 4; --cells 20 gives the 32,000-atom
+workload), rho 0.8, T* 0.8, seed 1003, LJ 12-6 rc 2.5 +
 Axilrod-Teller-Muto nu 0.073 over every triplet within rc (physical sign, -grad V),
 velocity Verlet dt 0.005, NVE, 10,000 steps, all in FP64 in the reference's operation
 order (oracle: vv_step0 -> update_container -> lj_c08 + atm_complete -> vv_step1).
 Writes E_total / N every 100 steps and the slope of a linear fit against time.
 
-  python tests/golden/make_drift_fp64.py [--steps 10000] [--threads 8]
+  python tests/golden/make_drift_fp64.py [--cells 10] [--steps 10000] [--threads 8] [--out FILE]
 """
 import argparse
 import json
@@ -29,10 +30,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--threads", type=int, default=8)
-    ap.add_argument("--out", default=os.path.join(HERE, "drift_fp64_fcc4000_atm.json"))
+    ap.add_argument("--cells", type=int, default=10)
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
     O.build()
-    s = S.fcc(10, 0.8, 0.8, 1003)
+    s = S.fcc(a.cells, 0.8, 0.8, 1003)
+    if a.out is None:
+        a.out = os.path.join(HERE, f"drift_fp64_fcc{s.n}_atm.json")
     n = s.n
 
     def forces(r):
@@ -56,7 +60,7 @@ def main():
                 print(f"step {k}  E/N {e[-1]:.10f}  {time.time() - t0:.0f}s", flush=True)
     t = np.array(steps) * DT
     slope = float(np.polyfit(t, np.array(e), 1)[0])
-    out = {"system": "4,000 argon atoms, FCC 10^3 x 4, rho 0.8, T* 0.8 (seed 1003), LJ rc 2.5 + ATM nu 0.073 "
+    out = {"system": f"{n:,} argon atoms, FCC {a.cells}^3 x 4, rho 0.8, T* 0.8 (seed 1003), LJ rc 2.5 + ATM nu 0.073 "
                      "(every triplet within rc, -grad V), dt 0.005, NVE",
            "precision": "FP64 throughout (oracle)", "every": EVERY, "steps": steps, "e_per_atom": e,
            "slope_e_per_atom_per_tau": slope, "max_dev_per_atom": float(np.max(np.abs(np.array(e) - e[0])))}

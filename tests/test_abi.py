@@ -31,3 +31,15 @@ def test_committed_surrogate_weights():
     p = np.load(DEFAULT_WEIGHTS)
     assert p.dtype == np.float32 and p.size == NPARAM and np.isfinite(p).all()
     assert NPARAM == 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48
+
+
+def test_max_neighbors_is_validated():
+    """The neighbour-list capacity is fixed at 96 sites: komd_create accepts 0 or 96 and
+    rejects any other value with KOMD_ERR_ARG (checked before any device call)."""
+    import pytest
+    from komd_amd import systems as S
+    s = S.fcc(5, 0.8, 0.8, 1)
+    for bad in (64, 128, -1):
+        with pytest.raises(K.KomdError, match="max_neighbors") as ei:
+            K.Context(s, max_neighbors=bad)
+        assert ei.value.status == 1

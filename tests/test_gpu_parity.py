@@ -268,16 +268,24 @@ def test_nonfinite_position_is_reported():
 
 @pytest.mark.parametrize("cells,seed", [(5, 51), (10, 52)])
 def test_atm_reference_mode(orc, cells, seed):
-    """Reference ATM_NOLIST semantics: +grad V on the C08 triplet set."""
+    """Reference ATM_NOLIST semantics: +grad V on the C08 triplet set.  The unique triplets
+    evaluated equal the oracle's count under the reference coverage (the cell-list walk
+    restricted to cells all equal or pairwise distinct, pinned to the C08 walk's forces
+    and the brute force's count)."""
     s = small(seed, cells)
+    print(f"atm ref cells={cells} n={s.n}", flush=True)
     r, b = orc.update_container(s)
     f_or, _ = orc.atm_c08(s, r, b, NU, sign=1.0, nthreads=8)
+    _, _, hits = orc.atm_complete(s, r, b, NU, coverage=1, sign=1.0, nthreads=8)
     with K.Context(s, nu=NU) as c:
         c.update_container()
+        c.counters(reset=True)
         c.forces_3b(K.ATM_REF)
         f = c.forces()
+        _, trip = c.counters()
+    assert trip == hits, "reference triplet set differs"
     mx, rms = rel_err(f, f_or)
-    print(f"ATM ref max {mx:.3e} rms {rms:.3e}")
+    print(f"ATM ref max {mx:.3e} rms {rms:.3e} triplets {hits}")
     assert mx < 1e-4 and rms < 1e-5
 
 

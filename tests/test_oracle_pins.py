@@ -252,6 +252,24 @@ def test_atm_complete_equals_brute_force():
     assert abs(es.sum() - eb) < 1e-10 * abs(eb)
 
 
+def test_atm_complete_reference_coverage_equals_brute_force():
+    """The cell-list ATM restricted to the reference coverage (cells all equal or pairwise
+    distinct) finds the brute force's triplets (count, forces, energy) and gives the C08
+    walk's forces: its count is the number of unique triplets the reference evaluates,
+    which the C08 walk's evaluation count is not (a triplet across a periodic face is
+    evaluated once per image combination holding an unshifted member)."""
+    for cells, seed in [(5, 5), (6, 6)]:
+        s = S.fcc(cells, 0.8, 1.0, seed, jitter=0.15)
+        r, b = O.update_container(s)
+        f, e, h = O.atm_complete(s, r, b, NU, coverage=1, sign=1.0)
+        fb, eb, hb = O.atm_brute(s, r, NU, site_cell=b.site_cell, coverage=1, sign=1.0)
+        fc, _ = O.atm_c08(s, r, b, NU, sign=1.0)
+        assert h == hb
+        assert np.abs(f - fb).max() < 1e-12 * np.abs(fb).max()
+        assert abs(e - eb) < 1e-12 * abs(eb)
+        assert np.abs(f - fc).max() < 1e-12 * np.abs(fc).max()
+
+
 def test_c08_triple_table_counts():
     """44 = 56 - 12 distinct-cell triples per 2x2x2 block: all 21 with the origin, and of the
     35 without it, the 12 lying in the x=1, y=1 or z=1 faces are left out."""
