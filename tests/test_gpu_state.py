@@ -49,8 +49,11 @@ def test_set_sites_with_more_sites_keeps_surrogate_buffers_sized():
     is dropped, and the results equal those of a fresh context."""
     from komd_amd.train import DEFAULT_WEIGHTS
     p = np.load(DEFAULT_WEIGHTS)
-    small = S.fcc(5, 0.8, 0.8, 311, jitter=0.05)
     big = S.fcc(8, 0.8, 0.8, 312, jitter=0.05)
+    small = big.copy()  # the same box (the context's grid), fewer sites
+    keep = np.sort(np.random.default_rng(311).choice(big.n, 900, replace=False))
+    for name in ("r", "v", "mass", "eps", "sig", "mol_id"):
+        setattr(small, name, getattr(big, name)[keep].copy())
     print(f"resize {small.n} -> {big.n}", flush=True)
     with K.Context(small, nu=NU) as c:
         c.surrogate_set_params(p)
@@ -113,7 +116,10 @@ def test_graphed_run_reports_list_overflow():
         with pytest.raises(K.KomdError) as ei:
             c.run(100, kind3b=K.ATM_EXACT)
         assert ei.value.status in (4, 5)    # overflow (or non-finite after capped forces)
-        ok = S.fcc(6, 0.8, 0.8, 46, jitter=0.02)
+        ok = dense.copy()  # the context's box, at density 0.79
+        keep = np.sort(np.random.default_rng(46).choice(dense.n, 400, replace=False))
+        for name in ("r", "v", "mass", "eps", "sig", "mol_id"):
+            setattr(ok, name, getattr(dense, name)[keep].copy())
         c.set_sites(ok)
         c.compute_forces(K.ATM_EXACT)
         c.run(100, kind3b=K.ATM_EXACT)
