@@ -53,8 +53,11 @@ enum {
     KOMD_3B_HYBRID = 4       /* exact ATM in a slab, surrogate elsewhere, cos^2 blend    */
 };
 
-/* Pair force kinds for komd_forces_pair. */
-enum { KOMD_PAIR_LJ12_6 = 1 };
+/* Pair force kinds for komd_forces_pair and komd_select_pair. */
+enum {
+    KOMD_PAIR_LJ12_6 = 1,   /* LJ 12-6, Lorentz-Berthelot mixing, same-molecule exclusion  */
+    KOMD_PAIR_TABLE = 2     /* tabulated radial force (the IBI force functor), no exclusion */
+};
 
 typedef struct {
     double low[3];       /* domain low corner  (config->domainLow)            */
@@ -77,7 +80,8 @@ typedef struct {
 typedef struct {
     double kinetic;        /* sum 0.5 m v.v over all sites                      */
     double temperature;    /* temp_factor * sum(m v.v)/(3 N_in_box)             */
-    double u_lj;           /* LJ 12-6 energy, unshifted (LJ12_6_Sensor.cpp:76)  */
+    double u_lj;           /* pair energy: LJ 12-6 unshifted (LJ12_6_Sensor.cpp:76), or the
+                              tabulated potential with KOMD_PAIR_TABLE selected (0 without one) */
     double u_3b;           /* three-body energy (exact ATM or surrogate)        */
     double u_bond;         /* harmonic bond energy                              */
     double u_angle;        /* harmonic angle energy                             */
@@ -151,6 +155,24 @@ komd_status komd_get_cells(komd_ctx* ctx, int64_t* site_cell, uint32_t* cell_cou
  * Adds to the forces.  Builds no neighbour list: komd_forces_3b builds the full lists
  * (every in-range neighbour of every site, capacity 96) that its kernels read. */
 komd_status komd_forces_pair(komd_ctx* ctx, int32_t kind);
+/* Tabulated pair force of KOMD_PAIR_TABLE: for every site pair within the cutoff
+ * (no molecule exclusion), F(r) from a piecewise-linear table and the force (r_i - r_j)/r F on
+ * i, its negative on j (IBI_Force::operator(), plugins/IBI/IBI_ForceFunctor.cpp:24-42).
+ * F(r) is FunctionPL::evaluateAt (plugins/IBI/IBI_Math.h:30-72): force_high above the last
+ * node, force_low below the first, otherwise linear interpolation on the segment whose upper
+ * node is the first node >= r (at r equal to the first node: the first value).  x: nodes,
+ * 2 to 2^20 of them, strictly increasing; force: values at the nodes.  potential (may be NULL)
+ * is the pair potential on the same nodes with its own defaults, used only for energies.
+ * Replaces FunctionPL::setXValues / setYValues / setLowerDefault / setUpperDefault on
+ * IBI_ForceFunctor::getForceFunction() and getPotentialFunction() (IBI_Math.h:15-32,
+ * IBI.cpp:61-69).  The table is copied. */
+komd_status komd_set_pair_table(komd_ctx* ctx, int64_t nodes, const double* x, const double* force,
+                                double force_low, double force_high, const double* potential, double potential_low,
+                                double potential_high);
+/* The pair force that komd_run, komd_compute_forces and komd_measure_observables apply:
+ * KOMD_PAIR_LJ12_6 (default) or KOMD_PAIR_TABLE (needs komd_set_pair_table), as the IBI
+ * plugin replaces the LJ functor in the registry (plugins/IBI/IBI.cpp:51-55). */
+komd_status komd_select_pair(komd_ctx* ctx, int32_t kind);
 /* Harmonic bond and angle forces (new; replaces the reference's FENE functor slot). */
 komd_status komd_forces_bonded(komd_ctx* ctx);
 /* Per-component clamp (Limit::handlePairList, Limit.cpp:19-25); no-op when limit_factor <= 0. */

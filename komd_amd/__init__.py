@@ -23,7 +23,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_STATE", 4: "ERR_OVERFLOW"
           6: "ERR_NOMEM"}
 BIN_COM, BIN_SITE = 0, 1
 ATM_EXACT, ATM_REF, SURROGATE, HYBRID = 1, 2, 3, 4
-PAIR_LJ12_6 = 1
+PAIR_LJ12_6, PAIR_TABLE = 1, 2
 SUR_NDESC, SUR_H1, SUR_H2 = 48, 64, 64
 SUR_NPARAM = 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48
 
@@ -74,6 +74,8 @@ _SIGNATURES = {
     "komd_update_container": (C.c_int, [_P]),
     "komd_get_cells": (C.c_int, [_P, _P, _P, _P, _P]),
     "komd_forces_pair": (C.c_int, [_P, C.c_int32]),
+    "komd_set_pair_table": (C.c_int, [_P, C.c_int64, _P, _P, C.c_double, C.c_double, _P, C.c_double, C.c_double]),
+    "komd_select_pair": (C.c_int, [_P, C.c_int32]),
     "komd_forces_bonded": (C.c_int, [_P]),
     "komd_forces_limit": (C.c_int, [_P]),
     "komd_forces_3b": (C.c_int, [_P, C.c_int32]),
@@ -275,8 +277,28 @@ class Context:
         self._check(self.lib.komd_get_cells(self._ctx, _p(site_cell), _p(count), _p(start), _p(srt)))
         return site_cell, count, start, srt
 
-    def forces_pair(self):
-        self._check(self.lib.komd_forces_pair(self._ctx, PAIR_LJ12_6))
+    def forces_pair(self, kind: int = PAIR_LJ12_6):
+        self._check(self.lib.komd_forces_pair(self._ctx, kind))
+
+    def set_pair_table(self, x, force, force_low: float = 0.0, force_high: float = 0.0, potential=None,
+                       potential_low: float = 0.0, potential_high: float = 0.0):
+        """Piecewise-linear pair force F(r) on nodes x (FunctionPL of the IBI functor) with
+        its values below the first and above the last node; optionally the potential on the
+        same nodes for energies."""
+        x = np.ascontiguousarray(x, np.float64)
+        f = np.ascontiguousarray(force, np.float64)
+        if f.shape != x.shape:
+            raise ValueError("force values must match the nodes")
+        u = None if potential is None else np.ascontiguousarray(potential, np.float64)
+        if u is not None and u.shape != x.shape:
+            raise ValueError("potential values must match the nodes")
+        self._check(self.lib.komd_set_pair_table(self._ctx, x.size, _p(x), _p(f), float(force_low),
+                                                 float(force_high), None if u is None else _p(u),
+                                                 float(potential_low), float(potential_high)))
+
+    def select_pair(self, kind: int):
+        """The pair force of run / compute_forces / observables: PAIR_LJ12_6 or PAIR_TABLE."""
+        self._check(self.lib.komd_select_pair(self._ctx, kind))
 
     def forces_bonded(self):
         self._check(self.lib.komd_forces_bonded(self._ctx))

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkomd_hip.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["komd_api.hip", "k_container.hip", "k_cell.hip", "k_bonded.hip", "k_surrogate.hip", "k_sensors.hip"]
+SOURCES = ["komd_api.hip", "k_container.hip", "k_cell.hip", "k_bonded.hip", "k_surrogate.hip", "k_sensors.hip",
+           "k_table.hip"]
 HOST = os.path.join(HERE, "host")
 HOST_SOURCES = ["komd_host.cpp", "main.cpp"]
 HOST_EXE = os.path.join(HERE, "komd_md")

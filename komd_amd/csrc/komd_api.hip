@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <vector>
@@ -181,6 +182,17 @@ int atm_code(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind == KOMD_3B
 // neighbour-list mode of a three-body kind: 0 none, 1 lists, 2 lists + cell directions
 // (the reference-mode ATM filter and the surrogate's per-cell force rows read them)
 int nl_mode(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind ? 2 : 0); }
+bool table_pair(const komd_ctx* c) { return c->pair_kind == KOMD_PAIR_TABLE; }
+// lists of a step: the three-body kind's, and at least plain lists for the tabulated pair force
+int lists_of(const komd_ctx* c, int kind3b) { return table_pair(c) ? std::max(1, nl_mode(kind3b)) : nl_mode(kind3b); }
+// The kernel that applies the second half-kick of an NVE step: the last force kernel that
+// completes a site's force (KICK_NONE: a separate k_kick after the forces).
+int kick_kernel(const komd_ctx* c, int kind3b) {
+    if (is_atm(kind3b)) return KICK_IN_ATM;
+    if (kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID) return KICK_IN_APPLY;
+    if (!table_pair(c)) return KICK_IN_CELL;
+    return c->prm.limit_factor > 0 ? KICK_NONE : KICK_IN_TABLE;  // the clamp follows the table pass
+}
 
 // Three-body forces on their own (ForceFunctor3B::operator(), ForceFunctor3B.cpp:10-14).
 hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
@@ -205,15 +217,23 @@ hipError_t forces_3b(komd_ctx* c, int kind, bool energy) {
 hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     if (bonded) KOMD_CHECK_HIP(launch_bonded(c, false));
     const int atm = atm_code(kind3b);
-    const int nl = nl_mode(kind3b);
+    const int nl = lists_of(c, kind3b);
     if (nl) KOMD_CHECK_HIP(ensure_nl(c, nl == 2));
     // uniform single-site LJ without the clamp: the three-body kernel sums the LJ force
     // from the site's compacted list (one pair per lane) instead of the pair pass's
     // candidate loop, where a lane's rare hits left most lanes idle
     // (the surrogate's descriptor pass likewise walks every site's list)
-    const bool lj_mv = c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+    const bool lj_mv = !table_pair(c) && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
     const bool lj_atm = atm && lj_mv, lj_sur = kind3b == KOMD_3B_SURROGATE && lj_mv;
-    KOMD_CHECK_HIP(launch_cell(c, !(lj_atm || lj_sur), nl, false, true));
+    if (table_pair(c)) {
+        // tabulated pair force (the IBI functor in LJ's registry slot, IBI.cpp:51-55) from
+        // the lists, then the clamp over the complete two-body force
+        KOMD_CHECK_HIP(launch_cell(c, false, nl, false, false));
+        KOMD_CHECK_HIP(launch_table(c, false));
+        KOMD_CHECK_HIP(launch_limit(c));
+    } else {
+        KOMD_CHECK_HIP(launch_cell(c, !(lj_atm || lj_sur), nl, false, true));
+    }
     if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr, lj_atm);
     if (kind3b == KOMD_3B_SURROGATE) {
         c->lj_in_desc = lj_sur;
@@ -234,17 +254,13 @@ hipError_t enqueue_step(komd_ctx* c, int kind3b, bool bonded, bool thermo) {
     KOMD_CHECK_HIP(launch_bin(c));
     // without the thermostat no kinetic sums are needed: the last force kernel of the
     // step applies the second half-kick to the sites whose force it completes
-    c->kick_in = thermo ? KICK_NONE
-                        : (is_atm(kind3b) ? KICK_IN_ATM
-                                          : (kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID ? KICK_IN_APPLY
-                                                                                                      : KICK_IN_CELL));
+    const int kick = thermo ? KICK_NONE : kick_kernel(c, kind3b);
+    c->kick_in = kick;
     const hipError_t e = enqueue_forces(c, kind3b, bonded);
     c->kick_in = KICK_NONE;
     KOMD_CHECK_HIP(e);
-    if (thermo) {
-        KOMD_CHECK_HIP(launch_kick(c, true));
-        KOMD_CHECK_HIP(launch_beta_only(c));  // beta for the next step's rescale
-    }
+    if (kick == KICK_NONE) KOMD_CHECK_HIP(launch_kick(c, thermo));
+    if (thermo) KOMD_CHECK_HIP(launch_beta_only(c));  // beta for the next step's rescale
     return hipSuccess;
 }
 
@@ -316,6 +332,7 @@ void komd_destroy(komd_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_graph(c);
     surrogate_free(c);
+    table_free(c);
     free_sites(c);
     if (c->d.scal) hipFree(c->d.scal);
     if (c->d.flags) hipFree(c->d.flags);
@@ -589,9 +606,47 @@ komd_status komd_get_cells(komd_ctx* c, int64_t* site_cell, uint32_t* cell_count
 komd_status komd_forces_pair(komd_ctx* c, int32_t kind) {
     komd_status s = need_sites(c);
     if (s) return s;
+    if (kind == KOMD_PAIR_TABLE) {
+        if (!c->tab) return fail(c, KOMD_ERR_STATE, "no pair table: call komd_set_pair_table first");
+        HIPCALL(c, ensure_nl(c, false), "komd_forces_pair");
+        HIPCALL(c, launch_cell(c, false, 1, false, false), "komd_forces_pair");
+        HIPCALL(c, launch_table(c, false), "komd_forces_pair");
+        return sync_check(c, "komd_forces_pair");
+    }
     if (kind != KOMD_PAIR_LJ12_6) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
     HIPCALL(c, launch_cell(c, true, 0, false, false), "komd_forces_pair");
     return sync_check(c, "komd_forces_pair");
+}
+
+komd_status komd_set_pair_table(komd_ctx* c, int64_t nodes, const double* x, const double* force, double force_low,
+                                double force_high, const double* potential, double potential_low,
+                                double potential_high) {
+    if (!c) return KOMD_ERR_ARG;
+    if (nodes < 2 || nodes > (1 << 20) || !x || !force)
+        return fail(c, KOMD_ERR_ARG, "pair table: 2 to 2^20 nodes, x and force required");
+    for (int64_t k = 0; k < nodes; k++) {
+        if (!std::isfinite(x[k]) || !std::isfinite(force[k]) || (potential && !std::isfinite(potential[k])))
+            return fail(c, KOMD_ERR_ARG, "pair table: non-finite node or value");
+        if (k && !(x[k] > x[k - 1])) return fail(c, KOMD_ERR_ARG, "pair table: nodes must increase strictly");
+    }
+    if (!std::isfinite(force_low) || !std::isfinite(force_high) || !std::isfinite(potential_low) ||
+        !std::isfinite(potential_high))
+        return fail(c, KOMD_ERR_ARG, "pair table: non-finite default value");
+    if (c->stream) HIPCALL(c, hipStreamSynchronize(c->stream), "komd_set_pair_table");
+    drop_graph(c);  // a captured step holds the old table's addresses
+    HIPCALL(c, table_upload(c, nodes, x, force, force_low, force_high, potential, potential_low, potential_high),
+            "komd_set_pair_table");
+    return KOMD_OK;
+}
+
+komd_status komd_select_pair(komd_ctx* c, int32_t kind) {
+    if (!c) return KOMD_ERR_ARG;
+    if (kind != KOMD_PAIR_LJ12_6 && kind != KOMD_PAIR_TABLE) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
+    if (kind == KOMD_PAIR_TABLE && !c->tab)
+        return fail(c, KOMD_ERR_STATE, "no pair table: call komd_set_pair_table first");
+    if (kind != c->pair_kind) drop_graph(c);
+    c->pair_kind = kind;
+    return KOMD_OK;
 }
 
 komd_status komd_forces_bonded(komd_ctx* c) {
@@ -676,8 +731,7 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
     if (thermo) HIPCALL(c, launch_run_scalars(c, t0), "komd_run");
     {
         // device buffers of the chosen passes exist before any graph capture (no allocation while capturing)
-        const int atm = atm_code(kind3b);
-        const int nl = nl_mode(kind3b);
+        const int nl = lists_of(c, kind3b);
         if (nl) HIPCALL(c, ensure_nl(c, nl == 2), "komd_run");
         if (kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID) {
             hipError_t e = surrogate_prepare(c, kind3b == KOMD_3B_HYBRID);
@@ -696,7 +750,7 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
             }
         }
     } else {
-        const int key = kind3b * 4 + (bonded ? 2 : 0) + (thermo ? 1 : 0);
+        const int key = (c->pair_kind * 8 + kind3b) * 4 + (bonded ? 2 : 0) + (thermo ? 1 : 0);
         if (!c->graph_exec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t graph = nullptr;
@@ -837,12 +891,20 @@ komd_status measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* o
         return KOMD_OK;
     };
     const int atm = atm_code(kind3b);
-    const int nl = nl_mode(kind3b);
+    const int nl = lists_of(c, kind3b);
     if (nl) HIPCALL(c, ensure_nl(c, nl == 2), "komd_measure_observables");
-    HIPCALL(c, launch_cell(c, true, nl, true, false), "komd_measure_observables");
-    if ((s = reduce(d.red, cell_blocks(c), 1))) return s;
-    out->u_lj = vals[0];
-    if ((s = add_virial(cell_blocks(c)))) return s;
+    if (table_pair(c)) {
+        HIPCALL(c, launch_cell(c, false, nl, false, false), "komd_measure_observables");
+        HIPCALL(c, launch_table(c, true), "komd_measure_observables");
+        if ((s = reduce(d.red, table_blocks(c), 1))) return s;
+        out->u_lj = vals[0];
+        if ((s = add_virial(table_blocks(c)))) return s;
+    } else {
+        HIPCALL(c, launch_cell(c, true, nl, true, false), "komd_measure_observables");
+        if ((s = reduce(d.red, cell_blocks(c), 1))) return s;
+        out->u_lj = vals[0];
+        if ((s = add_virial(cell_blocks(c)))) return s;
+    }
     if (atm) {
         HIPCALL(c, launch_atm(c, atm, true, nullptr, nullptr, nullptr), "komd_measure_observables");
         if ((s = reduce(d.red, atm_blocks(c), 1))) return s;

@@ -1,6 +1,7 @@
 // Capacities of the cell-tile force kernel (k_cell.hip).
 #pragma once
 #include "komd_internal.hpp"
+#include "komd_reduce.hpp"
 
 namespace komd {
 constexpr int kCellThreads = 256;  // one workgroup (4 waves) per home cell
@@ -11,4 +12,56 @@ constexpr int kChunkLJ = 768;      // staging chunk of the pair-only pass (any n
 constexpr int kMaxOwn = 64;        // home sites per cell when the three-body pass runs
 constexpr int kFullCap = 96;       // full neighbour list capacity per home site (three-body pass)
 constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row
+
+// ---- helpers shared by the list-driven kernels (k_cell.hip, k_table.hip)
+
+__device__ __forceinline__ double image_shift(uint32_t code, double L) {
+    return code == 1u ? L : (code == 2u ? -L : 0.0);
+}
+
+// |(r_a + s_a) - (r_b + s_b)|^2 in FP64, reference operation order, no contraction
+// (the pragma covers the operators written here; __dmul_rn / __dadd_rn would not be
+// covered, their bodies are plain * and + compiled with contraction on).
+__device__ __forceinline__ double exact_r2(const double* __restrict__ rx, const double* __restrict__ ry,
+                                           const double* __restrict__ rz, const Grid& g, uint32_t wa, uint32_t wb) {
+#pragma clang fp contract(off)
+    const uint32_t i = wa & kIdxMask, j = wb & kIdxMask, ca = wa >> kIdxBits, cb = wb >> kIdxBits;
+    const double xa = rx[i] + image_shift(ca & 3u, g.size[0]);
+    const double ya = ry[i] + image_shift((ca >> 2) & 3u, g.size[1]);
+    const double za = rz[i] + image_shift((ca >> 4) & 3u, g.size[2]);
+    const double xb = rx[j] + image_shift(cb & 3u, g.size[0]);
+    const double yb = ry[j] + image_shift((cb >> 2) & 3u, g.size[1]);
+    const double zb = rz[j] + image_shift((cb >> 4) & 3u, g.size[2]);
+    const double dx = xa - xb, dy = ya - yb, dz = za - zb;
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
+// Sum over the wave in a fixed order, the total in every lane: quads by quad_perm,
+// 16-lane rows by row_ror (DPP, no LDS), then the four row sums read out.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x124>(v);  // row_ror:4
+    v += dpp_f64<0x128>(v);  // row_ror:8
+    auto row = [&](int l) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                                __builtin_amdgcn_readlane(__double2loint(v), l));
+    };
+    return (row(0) + row(16)) + (row(32) + row(48));
+}
+
+// the lanes of the wave whose predicate holds (the builtin takes the predicate as a
+// lane mask; the int-taking __ballot rebuilds it from a VGPR)
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// LDS writes of this wave visible to this wave's later reads.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 }  // namespace komd

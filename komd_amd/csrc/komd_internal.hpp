@@ -58,6 +58,7 @@ struct Partials {
 };
 
 struct Surrogate;  // k_surrogate.hip
+struct PairTable;  // k_table.hip
 
 struct DevState {
     // sites
@@ -134,6 +135,9 @@ struct komd_ctx {
     int graph_key = -1;
     // surrogate
     komd::Surrogate* sur = nullptr;
+    // pair force of komd_run / komd_compute_forces / komd_measure_observables
+    int pair_kind = KOMD_PAIR_LJ12_6;
+    komd::PairTable* tab = nullptr;  // KOMD_PAIR_TABLE (komd_set_pair_table)
     // pinned host slots the flag snapshots of komd_run land in, and their events
     uint32_t* flag_host = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
@@ -204,7 +208,7 @@ hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins);
 // the full lists, 2 lists + cell directions.  launch_atm: per-site ATM from the lists;
 // atm: 1 physical, 2 reference; lam/dlam: hybrid weights (nullptr otherwise); esite:
 // per-site energy share out (optional).
-enum { KICK_NONE = 0, KICK_IN_CELL = 1, KICK_IN_ATM = 2, KICK_IN_APPLY = 3 };
+enum { KICK_NONE = 0, KICK_IN_CELL = 1, KICK_IN_ATM = 2, KICK_IN_APPLY = 3, KICK_IN_TABLE = 4 };
 hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit);
 // lj: the kernel also adds the LJ force of every listed pair (uniform single-site
 // systems without the force clamp; the pair pass then skips LJ)
@@ -212,6 +216,13 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
                       bool lj = false);
 int cell_blocks(komd_ctx* c);
 int atm_blocks(komd_ctx* c);
+// k_table.hip: tabulated pair force over current neighbour lists; with energy the pair
+// energy and virial partials are left in d.red / d.vpart, table_blocks of them
+hipError_t launch_table(komd_ctx* c, bool energy);
+int table_blocks(komd_ctx* c);
+hipError_t table_upload(komd_ctx* c, int64_t nodes, const double* x, const double* y, double y_low, double y_high,
+                        const double* u, double u_low, double u_high);
+void table_free(komd_ctx* c);
 // k_bonded.hip
 hipError_t launch_bonded(komd_ctx* c, bool energy);
 hipError_t launch_limit(komd_ctx* c);

@@ -59,6 +59,9 @@ def lib():
             "or_num_threads": (i32, []),
             "or_virial": (None, [i64, P, P, P, P, f64, P, f64, P, i32, i64, P, f64, f64, i64, P, f64, f64, P]),
             "or_rdf": (None, [i64, P, P, f64, f64, i64, P, i32]),
+            "or_table_eval": (f64, [f64, i64, P, P, f64, f64]),
+            "or_table_c08": (None, [i64, P, f64, P, P, P, P, P, i64, P, P, f64, f64, P, f64, f64, P, P, P, i32]),
+            "or_table_brute": (None, [i64, P, f64, P, i64, P, P, f64, f64, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -226,6 +229,50 @@ def rdf(com, box, max_r, delta_r, nbins, nthreads=8):
     lib().or_rdf(com.shape[0], _p(com), _p(_c(box, np.float64)), float(max_r), float(delta_r), int(nbins), _p(out),
                  int(nthreads))
     return out
+
+
+class Table:
+    """A piecewise-linear function as the reference's FunctionPL holds it (IBI_Math.h:13-41):
+    nodes x (ascending), values y, and the values returned below the first and above the
+    last node."""
+
+    def __init__(self, x, y, low=0.0, high=0.0):
+        self.x = _c(x, np.float64)
+        self.y = _c(y, np.float64)
+        self.low = float(low)
+        self.high = float(high)
+
+    def __call__(self, r):
+        L = lib()
+        r = np.atleast_1d(np.asarray(r, np.float64))
+        out = np.array([L.or_table_eval(float(v), self.x.size, _p(self.x), _p(self.y), self.low, self.high)
+                        for v in r.ravel()])
+        return out.reshape(r.shape)
+
+
+def table_c08(sys, r, b: Binning, force: Table, cutoff=2.5, potential: Table | None = None, nthreads=1):
+    """Tabulated pair forces (IBI_ForceFunctor.cpp:24-42) over the reference pair set; with a
+    potential table on the same nodes also the total energy and the virial tensor
+    (XX XY YY XZ YZ ZZ), each physical pair once.  Returns (f, u, w)."""
+    r = _c(r, np.float64)
+    f = np.zeros_like(r)
+    u = np.zeros(1)
+    w = np.zeros(6)
+    pu = _c(potential.y, np.float64) if potential is not None else None
+    lib().or_table_c08(r.shape[0], _p(r), float(cutoff), _p(_c(sys.low, np.float64)), _p(_c(sys.high, np.float64)),
+                       _p(b.nd), _p(b.cell_start), _p(b.cell_list), force.x.size, _p(force.x), _p(force.y), force.low,
+                       force.high, _p(pu) if pu is not None else None,
+                       potential.low if potential is not None else 0.0,
+                       potential.high if potential is not None else 0.0, _p(f), _p(u), _p(w), int(nthreads))
+    return f, float(u[0]), w
+
+
+def table_brute(sys, r, force: Table, cutoff=2.5):
+    r = _c(r, np.float64)
+    f = np.zeros_like(r)
+    lib().or_table_brute(r.shape[0], _p(r), float(cutoff), _p(_c(sys.box, np.float64)), force.x.size, _p(force.x),
+                         _p(force.y), force.low, force.high, _p(f))
+    return f
 
 
 def limit(sys, f, limit_factor):

@@ -1092,3 +1092,161 @@ EXPORT void or_rdf(int64_t m, const double* com, const double* box, double max_r
         free(loc);
     }
 }
+
+/* ------------------------------------------------------------------ tabulated pair force (IBI) */
+
+/* FunctionPL::evaluateAt (plugins/IBI/IBI_Math.h:30-41) with findUpperNode (:59-67) and
+ * linearInterpolation (:49-56): def_high above the last node, def_low below the first,
+ * otherwise ya + (yb - ya) / (xb - xa) * (x - xa) on the segment whose upper node is the
+ * first node >= x.  At x equal to the first node the reference's lower index is -1 (a
+ * read before the table); this restatement returns the first node's value there, the
+ * limit of the first segment. */
+EXPORT double or_table_eval(double x, int64_t n, const double* xv, const double* yv, double def_low, double def_high) {
+    if (x > xv[n - 1]) return def_high;
+    if (x < xv[0]) return def_low;
+    int64_t i = 0;
+    while (x > xv[i]) {
+        i++;
+        if (i == n) break;
+    }
+    if (i == 0) return yv[0];
+    const double ya = yv[i - 1], yb = yv[i], xa = xv[i - 1], xb = xv[i];
+    return ya + (yb - ya) / (xb - xa) * (x - xa);
+}
+
+typedef void (*pair_fn)(int64_t i0, int64_t i1, const double* s0, const double* s1, void* ctx);
+
+/* The reference's pair set (the walk of or_lj_c08: 13 C08 cell pairs per base cell, then
+ * the intra-cell pairs of domain cells), one callback per candidate. */
+static void c08_pair_walk(const int64_t* nd, const double* low, const double* high, const int64_t* cell_start,
+                          const int64_t* cell_list, pair_fn fn, void* ctx, int nthreads) {
+    int64_t na[3] = {nd[0] + 2, nd[1] + 2, nd[2] + 2};
+    double dom[3] = {high[0] - low[0], high[1] - low[1], high[2] - low[2]};
+    int64_t nb = c08_bases(na, NULL);
+    int64_t* bases = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)nb);
+    c08_bases(na, bases);
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+    for (int64_t b = 0; b < nb; b++) {
+        for (int o = 0; o < 13; o++) {
+            int64_t c0[3], c1[3];
+            for (int d = 0; d < 3; d++) {
+                c0[d] = bases[3 * b + d] + PAIR_OFF[o][0][d];
+                c1[d] = bases[3 * b + d] + PAIR_OFF[o][1][d];
+            }
+            double s0[3], s1[3];
+            cell_shift(c0, na, dom, s0);
+            cell_shift(c1, na, dom, s1);
+            if (nonzero3(s0) && nonzero3(s1)) continue; /* checkState LinkedCells.cpp:471-474 */
+            int64_t f0 = c0[0] + na[0] * (c0[1] + na[1] * c0[2]);
+            int64_t f1 = c1[0] + na[0] * (c1[1] + na[1] * c1[2]);
+            int64_t n0 = cell_start[f0 + 1] - cell_start[f0], n1 = cell_start[f1 + 1] - cell_start[f1];
+            for (int64_t local = 0; local < n0 * n1; local++) {
+                int64_t i = local % n0, j = local / n0;
+                fn(cell_list[cell_start[f0] + i], cell_list[cell_start[f1] + j], s0, s1, ctx);
+            }
+        }
+    }
+    int64_t ncell = na[0] * na[1] * na[2];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+    for (int64_t c = 0; c < ncell; c++) {
+        int64_t x = c % na[0], y = (c / na[0]) % na[1], z = c / (na[0] * na[1]);
+        if (x == 0 || y == 0 || z == 0 || x == na[0] - 1 || y == na[1] - 1 || z == na[2] - 1) continue;
+        int64_t nc = cell_start[c + 1] - cell_start[c];
+        const double zero[3] = {0, 0, 0};
+        for (int64_t local = 0; local < nc * (nc - 1) / 2; local++) {
+            int64_t i = (int64_t)floor((-1 + sqrt(1 + 8 * (double)local)) / 2.0);
+            int64_t j = local - i * (i + 1) / 2;
+            i += 1;
+            fn(cell_list[cell_start[c] + i], cell_list[cell_start[c] + j], zero, zero, ctx);
+        }
+    }
+    free(bases);
+}
+
+typedef struct {
+    const double* r;
+    double cutoff2;
+    int64_t nodes;
+    const double *x, *y, *u; /* force table, optional potential table on the same nodes */
+    double y_low, y_high, u_low, u_high;
+    double* f;
+    double* u_total; /* optional */
+    double* vir;     /* optional, [6] XX XY YY XZ YZ ZZ */
+    int atomic;
+} table_ctx;
+
+static inline void add_to(double* p, double v, int atomic) {
+    if (atomic) {
+#pragma omp atomic
+        *p += v;
+    } else *p += v;
+}
+
+/* IBI_Force::operator() (plugins/IBI/IBI_ForceFunctor.cpp:24-42) on one candidate pair:
+ * dr = (r0 + s0) - (r1 + s1), skipped when |dr|^2 > cutoff^2, F = evaluateAt(|dr|),
+ * force = (dr / |dr|) F added to the unshifted first member and subtracted from the
+ * unshifted second.  No molecule exclusion (the functor has none).  Energy and virial
+ * (not in the reference functor) count each physical pair once, as lj_pair does. */
+static void table_pair(int64_t i0, int64_t i1, const double* s0, const double* s1, void* p) {
+    table_ctx* t = (table_ctx*)p;
+    double dr[3];
+    for (int d = 0; d < 3; d++) dr[d] = (t->r[3 * i0 + d] + s0[d]) - (t->r[3 * i1 + d] + s1[d]);
+    double dr2 = 0;
+    for (int d = 0; d < 3; d++) dr2 += dr[d] * dr[d];
+    if (dr2 > t->cutoff2) return;
+    const double rr = sqrt(dr2);
+    const double F = or_table_eval(rr, t->nodes, t->x, t->y, t->y_low, t->y_high);
+    double force[3];
+    for (int d = 0; d < 3; d++) force[d] = (dr[d] / rr) * F;
+    const int u0 = !nonzero3(s0), u1 = !nonzero3(s1);
+    for (int d = 0; d < 3; d++) {
+        if (u0) add_to(&t->f[3 * i0 + d], force[d], t->atomic);
+        if (u1) add_to(&t->f[3 * i1 + d], -force[d], t->atomic);
+    }
+    if (!u1) return;
+    if (t->u_total && t->u)
+        add_to(t->u_total, or_table_eval(rr, t->nodes, t->x, t->u, t->u_low, t->u_high), t->atomic);
+    if (t->vir) {
+        const double w[6] = {dr[0] * force[0], dr[0] * force[1], dr[1] * force[1],
+                             dr[0] * force[2], dr[1] * force[2], dr[2] * force[2]};
+        for (int k = 0; k < 6; k++) add_to(&t->vir[k], w[k], t->atomic);
+    }
+}
+
+/* The tabulated pair force over the reference's pair set; f accumulated. */
+EXPORT void or_table_c08(int64_t n, const double* r, double cutoff, const double* low, const double* high,
+                         const int64_t* nd, const int64_t* cell_start, const int64_t* cell_list, int64_t nodes,
+                         const double* x, const double* y, double y_low, double y_high, const double* u, double u_low,
+                         double u_high, double* f, double* u_total, double* vir, int nthreads) {
+    (void)n;
+    table_ctx t = {r, pow(cutoff, 2), nodes, x, y, u, y_low, y_high, u_low, u_high, f, u_total, vir, nthreads > 1};
+    c08_pair_walk(nd, low, high, cell_start, cell_list, table_pair, &t, nthreads);
+}
+
+/* Minimum-image brute force of the same force (every pair within the cutoff, no
+ * exclusion), FP64: the independent pin of the walk on small systems. */
+EXPORT void or_table_brute(int64_t n, const double* r, double cutoff, const double* box, int64_t nodes,
+                           const double* x, const double* y, double y_low, double y_high, double* f) {
+    const double rc2 = cutoff * cutoff;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++) {
+            double d[3], d2 = 0;
+            for (int k = 0; k < 3; k++) {
+                d[k] = r[3 * i + k] - r[3 * j + k];
+                d[k] -= box[k] * nearbyint(d[k] / box[k]);
+                d2 += d[k] * d[k];
+            }
+            if (d2 > rc2) continue;
+            const double rr = sqrt(d2);
+            const double F = or_table_eval(rr, nodes, x, y, y_low, y_high);
+            for (int k = 0; k < 3; k++) {
+                f[3 * i + k] += d[k] / rr * F;
+                f[3 * j + k] -= d[k] / rr * F;
+            }
+        }
+}
