@@ -202,6 +202,15 @@ komd_status komd_measure_observables(komd_ctx* ctx, int32_t kind3b, komd_observa
  * RDFSensor::write's averaging and g(r) normalisation stay with the caller. */
 komd_status komd_rdf(komd_ctx* ctx, double max_r, double delta_r, int64_t nbins, uint64_t* counts);
 
+/* komd_rdf without any allocation, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The molecule centres and the counts live in a static
+ * device array of 6.1 MiB that each GPU the library runs on holds until the process ends:
+ * at most 262,144 molecules (more: KOMD_ERR_ARG) and 8,192 bins.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (contexts on several host
+ * threads or streams) race; calls in sequence do not.  Results equal komd_rdf's. */
+komd_status komd_rdf_noalloc(komd_ctx* ctx, double max_r, double delta_r, int64_t nbins, uint64_t* counts);
+
 /* Simulation::run main loop (Simulation.cpp:49-74) on the device: nsteps x
  * {integrate0, update_container, LJ, [bonded], [limit], [3-body], integrate1,
  *  [temperature + rescale to t0 when t0 > 0]}.  kind3b: 0 none, else KOMD_3B_*.

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "komd_rescale": (C.c_int, [_P, C.c_double]),
     "komd_measure_observables": (C.c_int, [_P, C.c_int32, _P]),
     "komd_rdf": (C.c_int, [_P, C.c_double, C.c_double, C.c_int64, _P]),
+    "komd_rdf_noalloc": (C.c_int, [_P, C.c_double, C.c_double, C.c_int64, _P]),
     "komd_run": (C.c_int, [_P, C.c_int64, C.c_int32, C.c_int32, C.c_double]),
     "komd_compute_forces": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "komd_timing_enable": (C.c_int, [_P, C.c_int32]),
@@ -341,6 +342,14 @@ class Context:
         nbins = rdf_bins(max_r, delta_r)
         out = np.zeros(nbins, np.uint64)
         self._check(self.lib.komd_rdf(self._ctx, float(max_r), float(delta_r), nbins, _p(out)))
+        return out
+
+    def rdf_noalloc(self, max_r: float = 5.0, delta_r: float = 0.1) -> np.ndarray:
+        """The same sample as rdf() from the GPU's static scratch (komd_rdf_noalloc: no
+        device allocation; at most 262,144 molecules)."""
+        nbins = rdf_bins(max_r, delta_r)
+        out = np.zeros(nbins, np.uint64)
+        self._check(self.lib.komd_rdf_noalloc(self._ctx, float(max_r), float(delta_r), nbins, _p(out)))
         return out
 
     # -- measurement

@@ -15,6 +15,8 @@
 // values the reference's expressions give.
 #pragma clang fp contract(off)
 
+#include <mutex>
+
 #include "komd_internal.hpp"
 
 namespace komd {
@@ -22,6 +24,17 @@ namespace {
 
 constexpr int kRdfThreads = 256;  // one row molecule per thread
 constexpr int kRdfTile = 256;     // molecules per tile side
+
+// Scratch of komd_rdf_noalloc: the molecule centres (x, y, z blocks of kRdfScratchMol
+// doubles) and then the kRdfMaxBins u64 counts, 6.1 MiB in all.  Each GPU the library runs
+// on holds its own copy until the process ends; host code reaches it only through the
+// per-device address from hipGetSymbolAddress (g_scratch below), never by this name.
+constexpr size_t kRdfScratchWords = 3 * (size_t)kRdfScratchMol + kRdfMaxBins;
+static __device__ __align__(256) double scratch[kRdfScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};  // device address of `scratch` per device ordinal
 
 // com = (sum_s m_s r_s) / (sum_s m_s) over the molecule's sites in order.
 __global__ __launch_bounds__(256) void k_mol_com(int nmol, const uint32_t* __restrict__ ms,
@@ -56,6 +69,10 @@ struct RdfArgs {
 
 // delta - L round(delta / L) (RDFSensor.cpp:65), round half away from zero.
 __device__ __forceinline__ double min_image(double d, double L) { return d - L * round(d / L); }
+
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0ull;
+}
 
 __global__ __launch_bounds__(kRdfThreads) void k_rdf(RdfArgs a) {
     extern __shared__ unsigned int hist[];  // [nbins]
@@ -97,26 +114,20 @@ __global__ __launch_bounds__(kRdfThreads) void k_rdf(RdfArgs a) {
         if (hist[k]) atomicAdd(&a.out[k], (unsigned long long)hist[k]);
 }
 
-}  // namespace
-
-hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins) {
+// One sample into the given buffers: centres [nmol] x 3, counts [nbins] (cleared here by a
+// fill kernel, so the sequence holds no memset node when captured).
+hipError_t rdf_sample(komd_ctx* c, double max_r, double delta_r, int nbins, double* comx, double* comy, double* comz,
+                      unsigned long long* counts) {
     auto& d = c->d;
     const int nmol = (int)c->nmol;
-    if (!d.comx) {
-        const size_t bytes = (size_t)std::max(nmol, 1) * sizeof(double);
-        KOMD_CHECK_HIP(hipMalloc((void**)&d.comx, bytes));
-        KOMD_CHECK_HIP(hipMalloc((void**)&d.comy, bytes));
-        KOMD_CHECK_HIP(hipMalloc((void**)&d.comz, bytes));
-        KOMD_CHECK_HIP(hipMalloc((void**)&d.rdf, (size_t)kRdfMaxBins * sizeof(unsigned long long)));
-    }
-    ScopedTimer tm(c, "rdf");
-    KOMD_CHECK_HIP(hipMemsetAsync(d.rdf, 0, (size_t)nbins * sizeof(unsigned long long), c->stream));
+    k_zero_u64<<<(nbins + 255) / 256, 256, 0, c->stream>>>(counts, nbins);
+    KOMD_CHECK_HIP(hipGetLastError());
     if (nmol < 2) return hipSuccess;
     const uint32_t* ms = c->multisite ? d.mol_start : nullptr;
-    k_mol_com<<<(nmol + 255) / 256, 256, 0, c->stream>>>(nmol, ms, d.rx, d.ry, d.rz, d.mass, d.comx, d.comy, d.comz);
+    k_mol_com<<<(nmol + 255) / 256, 256, 0, c->stream>>>(nmol, ms, d.rx, d.ry, d.rz, d.mass, comx, comy, comz);
     KOMD_CHECK_HIP(hipGetLastError());
     RdfArgs a{};
-    a.cx = d.comx; a.cy = d.comy; a.cz = d.comz; a.out = d.rdf;
+    a.cx = comx; a.cy = comy; a.cz = comz; a.out = counts;
     a.nmol = nmol;
     a.ntile = (nmol + kRdfTile - 1) / kRdfTile;
     a.nbins = nbins;
@@ -129,6 +140,52 @@ hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins) {
     const long long grid = std::max<long long>(std::min<long long>(slots, 4096), (slots + 4095) / 4096);
     k_rdf<<<(unsigned)grid, kRdfThreads, (size_t)nbins * sizeof(unsigned int), c->stream>>>(a);
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins) {
+    auto& d = c->d;
+    if (!d.comx) {
+        const size_t bytes = (size_t)std::max<int64_t>(c->nmol, 1) * sizeof(double);
+        KOMD_CHECK_HIP(hipMalloc((void**)&d.comx, bytes));
+        KOMD_CHECK_HIP(hipMalloc((void**)&d.comy, bytes));
+        KOMD_CHECK_HIP(hipMalloc((void**)&d.comz, bytes));
+        KOMD_CHECK_HIP(hipMalloc((void**)&d.rdf, (size_t)kRdfMaxBins * sizeof(unsigned long long)));
+    }
+    ScopedTimer tm(c, "rdf");
+    return rdf_sample(c, max_r, delta_r, nbins, d.comx, d.comy, d.comz, d.rdf);
+}
+
+// The same sample in the static scratch (nmol <= kRdfScratchMol, checked by the caller):
+// no allocation, counts left at rdf_scratch_counts(c).
+hipError_t launch_rdf_noalloc(komd_ctx* c, double max_r, double delta_r, int nbins) {
+    double* base = c->rdf_scratch;
+    if (!base) return hipErrorInvalidSymbol;
+    return rdf_sample(c, max_r, delta_r, nbins, base, base + kRdfScratchMol, base + 2 * (size_t)kRdfScratchMol,
+                      rdf_scratch_counts(c));
+}
+
+unsigned long long* rdf_scratch_counts(komd_ctx* c) {
+    return c->rdf_scratch ? reinterpret_cast<unsigned long long*>(c->rdf_scratch + 3 * (size_t)kRdfScratchMol)
+                          : nullptr;
+}
+
+// Device address of `scratch` on the current device, looked up once per device and kept
+// (komd_create calls this with the context's device current).
+hipError_t rdf_scratch_init(komd_ctx* c) {
+    int dev = 0;
+    KOMD_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    c->rdf_scratch = g_scratch[dev];
+    return hipSuccess;
 }
 
 }  // namespace komd

@@ -322,6 +322,14 @@ komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx
     if (e == hipSuccess) e = hipMemset(c->d.scal, 0, 32 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->flag_host, 2 * sizeof(uint32_t));
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&c->flag_ev[k], hipEventDisableTiming);
+    if (e == hipSuccess) {
+        // this device's static RDF scratch; a failed lookup only disables komd_rdf_noalloc
+        const hipError_t es = rdf_scratch_init(c);
+        if (es != hipSuccess) {
+            c->rdf_scratch_err = hipGetErrorString(es);
+            (void)hipGetLastError();  // not left for the next launch check to report
+        }
+    }
     *out = c;
     if (e != hipSuccess) return hip_fail(c, e, "komd_create");
     return KOMD_OK;
@@ -975,7 +983,11 @@ komd_status komd_measure_observables(komd_ctx* c, int32_t kind3b, komd_observabl
     return s2;
 }
 
-komd_status komd_rdf(komd_ctx* c, double max_r, double delta_r, int64_t nbins, uint64_t* counts) {
+}  // extern "C"
+
+namespace {
+// Arguments of one RDF sample (RDFSensor.cpp:19: nbins = (uint64)(max_r / delta_r)).
+komd_status rdf_args(komd_ctx* c, double max_r, double delta_r, int64_t nbins, const uint64_t* counts) {
     komd_status s = need_sites(c);
     if (s) return s;
     if (!counts) return fail(c, KOMD_ERR_ARG, "rdf: counts is NULL");
@@ -984,6 +996,30 @@ komd_status komd_rdf(komd_ctx* c, double max_r, double delta_r, int64_t nbins, u
     if (!(q >= 1.0) || !(q < (double)kRdfMaxBins + 1.0) || (int64_t)(uint64_t)q != nbins)
         return fail(c, KOMD_ERR_ARG,
                     "rdf: nbins must equal (uint64)(max_r / delta_r), between 1 and " + std::to_string(kRdfMaxBins));
+    return KOMD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+komd_status komd_rdf_noalloc(komd_ctx* c, double max_r, double delta_r, int64_t nbins, uint64_t* counts) {
+    komd_status s = rdf_args(c, max_r, delta_r, nbins, counts);
+    if (s) return s;
+    if (c->nmol > kRdfScratchMol)
+        return fail(c, KOMD_ERR_ARG, "rdf_noalloc: more than " + std::to_string(kRdfScratchMol) +
+                                         " molecules, the capacity of the static scratch");
+    if (!c->rdf_scratch) return fail(c, KOMD_ERR_HIP, "rdf_noalloc: no scratch address: " + c->rdf_scratch_err);
+    HIPCALL(c, launch_rdf_noalloc(c, max_r, delta_r, (int)nbins), "komd_rdf_noalloc");
+    s = sync_check(c, "komd_rdf_noalloc");
+    if (s) return s;
+    HIPCALL(c, hipMemcpy(counts, rdf_scratch_counts(c), (size_t)nbins * sizeof(uint64_t), hipMemcpyDeviceToHost),
+            "komd_rdf_noalloc");
+    return KOMD_OK;
+}
+
+komd_status komd_rdf(komd_ctx* c, double max_r, double delta_r, int64_t nbins, uint64_t* counts) {
+    komd_status s = rdf_args(c, max_r, delta_r, nbins, counts);
+    if (s) return s;
     HIPCALL(c, launch_rdf(c, max_r, delta_r, (int)nbins), "komd_rdf");
     s = sync_check(c, "komd_rdf");
     if (s) return s;

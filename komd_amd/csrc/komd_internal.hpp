@@ -97,6 +97,7 @@ struct DevState {
 };
 
 constexpr int kRdfMaxBins = 8192;  // u32 LDS histogram of 32 KiB per workgroup
+constexpr int kRdfScratchMol = 262144;  // molecules komd_rdf_noalloc's static scratch holds
 
 struct Timer {
     double total_ms = 0;
@@ -141,6 +142,9 @@ struct komd_ctx {
     // pinned host slots the flag snapshots of komd_run land in, and their events
     uint32_t* flag_host = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
+    // this device's static RDF scratch (komd_rdf_noalloc); null when its lookup failed
+    double* rdf_scratch = nullptr;
+    std::string rdf_scratch_err;
     // Integrator::Step1 fused into the last force kernel of an NVE step (no kinetic sums
     // are needed there): 0 none (k_kick runs), else the kernel that applies it
     int kick_in = 0;
@@ -204,6 +208,11 @@ hipError_t launch_kin_tensor(komd_ctx* c);      // sum m v_a v_b partials -> d.v
 constexpr int kKinBlocks = 256;
 // k_sensors.hip: one RDF sample of the molecules' COM distances into d.rdf (nbins counts)
 hipError_t launch_rdf(komd_ctx* c, double max_r, double delta_r, int nbins);
+// the same sample in the device's static scratch (no allocation); counts at rdf_scratch_counts
+hipError_t launch_rdf_noalloc(komd_ctx* c, double max_r, double delta_r, int nbins);
+unsigned long long* rdf_scratch_counts(komd_ctx* c);
+// looks up the static scratch's address on the current device (once per device)
+hipError_t rdf_scratch_init(komd_ctx* c);
 // k_cell.hip.  launch_cell: LJ (+ clamp) on cell tiles; nl: 0 no neighbour list, 1 write
 // the full lists, 2 lists + cell directions.  launch_atm: per-site ATM from the lists;
 // atm: 1 physical, 2 reference; lam/dlam: hybrid weights (nullptr otherwise); esite:
