@@ -183,7 +183,6 @@ __device__ __forceinline__ void radial_gauss(float y, float g[NR]) {
 constexpr int kLps = 4;
 // Sites of the 27-cell neighbourhood held by the surrogate force pass (about 394 at
 // density 0.8; a larger neighbourhood is reported as a cell overflow).
-constexpr int kSurM = 640;
 
 // ---------------------------------------------------------------------------
 // descriptors

@@ -121,8 +121,9 @@ komd_status sync_check(komd_ctx* c, const char* where) {
                             " in-range neighbours (the neighbour-list capacity)");
         if (flags & FLAG_CELL_OVERFLOW)
             return fail(c, KOMD_ERR_OVERFLOW,
-                        std::string(where) + ": a cell neighbourhood holds more than 768 sites or a cell more "
-                                             "than 64 (use a smaller cell_size for the three-body pass)");
+                        std::string(where) + ": surrogate force pass: a cell neighbourhood holds more than " +
+                            std::to_string(kSurM) + " sites or a cell more than " + std::to_string(kRows) +
+                            " (use a smaller cell_size)");
         if (flags & FLAG_NONFINITE)
             return fail(c, KOMD_ERR_NONFINITE, std::string(where) + ": non-finite position or velocity");
     }

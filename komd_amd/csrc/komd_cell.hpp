@@ -12,6 +12,7 @@ constexpr int kChunkLJ = 768;      // staging chunk of the pair-only pass (any n
 constexpr int kMaxOwn = 64;        // home sites per cell when the three-body pass runs
 constexpr int kFullCap = 96;       // full neighbour list capacity per home site (three-body pass)
 constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row
+constexpr int kSurM = 640;         // staged neighbourhood capacity of the surrogate force pass
 
 // ---- helpers shared by the list-driven kernels (k_cell.hip, k_table.hip)
 

@@ -93,6 +93,56 @@ def neighbour_pairs(r, box, rc=RC):
     return np.concatenate(I), np.concatenate(J), np.concatenate(Dv)
 
 
+def neighbour_pairs_cells(r, box, rc=RC, centers=None):
+    """The pairs of neighbour_pairs from a cell list (cells of edge >= rc, 27 neighbour cells):
+    the same pairs, the same FP64 displacements and the same (i, j) order, in O(N) work.
+    centers restricts i to the given sites.  Boxes under three cells per side fall back to
+    the brute force."""
+    r = np.asarray(r, np.float64)
+    box = np.asarray(box, np.float64)
+    n = r.shape[0]
+    nc = np.floor(box / rc).astype(np.int64)
+    if (nc < 3).any():
+        i, j, d = neighbour_pairs(r, box, rc)
+        if centers is None:
+            return i, j, d
+        keep = np.isin(i, np.asarray(centers))
+        return i[keep], j[keep], d[keep]
+    cs = box / nc
+    rw = r - box * np.floor(r / box)
+    ci = np.minimum((rw / cs).astype(np.int64), nc - 1)
+    flat = ci[:, 0] + nc[0] * (ci[:, 1] + nc[1] * ci[:, 2])
+    ncell = int(np.prod(nc))
+    cnt = np.bincount(flat, minlength=ncell)
+    order = np.argsort(flat, kind="stable")
+    start = np.concatenate([[0], np.cumsum(cnt)])
+    m = int(cnt.max())
+    table = np.full((ncell, m), -1, np.int64)          # the sites of every cell, padded with -1
+    slot = np.arange(n) - start[flat[order]]
+    table[flat[order], slot] = order
+    cen = np.arange(n, dtype=np.int64) if centers is None else np.unique(np.asarray(centers, np.int64))
+    I, J, Dv = [], [], []
+    for ox in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            for oz in (-1, 0, 1):
+                c = (ci[cen] + np.array([ox, oy, oz])) % nc
+                cand = table[c[:, 0] + nc[0] * (c[:, 1] + nc[1] * c[:, 2])]     # [centres, m]
+                ii = np.repeat(cen, m)
+                jj = cand.ravel()
+                ok = jj >= 0
+                ii, jj = ii[ok], jj[ok]
+                d = r[jj] - r[ii]
+                d -= box * np.rint(d / box)
+                r2 = (d * d).sum(-1)
+                keep = (r2 < rc * rc) & (ii != jj)
+                I.append(ii[keep])
+                J.append(jj[keep])
+                Dv.append(d[keep])
+    I, J, Dv = np.concatenate(I), np.concatenate(J), np.concatenate(Dv)
+    srt = np.lexsort((J, I))
+    return I[srt], J[srt], Dv[srt]
+
+
 # ---- basis functions ----------------------------------------------------------
 def _fc(r):
     return 0.5 * (np.cos(np.pi * r / RC) + 1.0), -0.5 * np.pi / RC * np.sin(np.pi * r / RC)
@@ -237,6 +287,30 @@ def energy_forces(p, n, i, j, d):
     return float(m["E"].sum()), m["E"], F
 
 
+def energy_forces_subset(p, r, box, sites, rc=RC):
+    """Raw descriptors, per-site energies and forces -dE/dr of the given sites only, FP64.
+    E_i needs the neighbours of i; F_i = sum_j (dE_i/dd_ij - dE_j/dd_ji) also needs the
+    states of i's neighbours j, so the pairs of every site within rc of the subset are
+    built (a cell list), the network runs on those sites, and the pair gradients only on
+    the subset's pairs.  Equals energy_forces restricted to the subset."""
+    r = np.asarray(r, np.float64)
+    n = r.shape[0]
+    sites = np.unique(np.asarray(sites, np.int64))
+    i0, j0, _ = neighbour_pairs_cells(r, box, rc, centers=sites)
+    ring = np.unique(np.concatenate([sites, j0]))
+    i, j, d = neighbour_pairs_cells(r, box, rc, centers=ring)
+    D, T = descriptors(n, i, d)
+    m = mlp(p, D)
+    A = _state(m["G"], T)
+    own = np.isin(i, sites)
+    Grad = m["G"][:, :NR]
+    c_own = _pair_grad(Grad, A, i[own], d[own])
+    c_nbr = _pair_grad(Grad, A, j[own], -d[own])
+    F = np.zeros((n, 3))
+    np.add.at(F, i[own], c_own - c_nbr)
+    return D[sites], m["E"][sites], F[sites]
+
+
 # ---- training loss and its gradient ------------------------------------------------
 def _jvp(T, n, i, j, d, delta):
     """dD_a along the displacement field delta: sum_j dD_a/dd_aj . (delta_j - delta_a)."""
@@ -335,7 +409,7 @@ def lam_of(x, axis_len, center, half, blend):
 
 def atm_triplets(r, box, rc=RC):
     """Every triplet i < j < k with all three minimum-image distances below rc."""
-    i, j, d = neighbour_pairs(r, box, rc)
+    i, j, d = neighbour_pairs_cells(r, box, rc)
     nb = [[] for _ in range(len(r))]
     for a, b in zip(i, j):
         nb[a].append(b)
@@ -394,6 +468,7 @@ def hybrid_energy_forces(p, r, box, nu, axis, center, half, blend):
     F = np.zeros((n, 3))
     # exact part
     trip = atm_triplets(r, box)
+    trip = trip[lam[trip].sum(1) > 0]   # triplets of zero weight add neither energy nor force
     E = 0.0
     if len(trip):
         V, gi, gj, gk = atm_triplet_terms(r, box, trip, nu)
@@ -404,7 +479,7 @@ def hybrid_energy_forces(p, r, box, nu, axis, center, half, blend):
             np.add.at(F[:, axis], trip[:, col], -V * dl[trip[:, col]] / 3.0)
     # surrogate part: site energies weighted by 1 - lambda
     cw = 1.0 - lam
-    i, j, d = neighbour_pairs(r, box)
+    i, j, d = neighbour_pairs_cells(r, box)
     D, T = descriptors(n, i, d)
     m = mlp(p, D)
     G = m["G"] * cw[:, None]

@@ -166,3 +166,39 @@ def test_hybrid_forces_are_minus_gradient():
             rm = r.copy(); rm[s, ax] -= h
             fd = -(SO.hybrid_energy_forces(p, rp, box, *args)[0] - SO.hybrid_energy_forces(p, rm, box, *args)[0]) / (2 * h)
             assert abs(fd - F[s, ax]) < 2e-6 * max(1.0, abs(F[s, ax])), (s, ax, fd, F[s, ax])
+
+
+@pytest.mark.parametrize("case", ["fcc", "slab", "chains"])
+def test_cell_list_neighbours_equal_brute_force(case):
+    """The O(N) cell-list search returns the brute force's pairs, FP64 displacements and
+    (i, j) order, for all sites and for a subset of centres (a missed neighbour cell, a
+    wrong periodic image or a double-counted cell would change them)."""
+    from komd_amd import systems as S
+    s = {"fcc": lambda: S.fcc(6, 0.8, 1.0, 21, jitter=0.2),
+         "slab": lambda: S.simple_cubic(24, 8, 8, 0.8, 1.0, 22),
+         "chains": lambda: S.chains((5, 5, 5), 4, 0.7, seed=23)}[case]()
+    box = np.asarray(s.box, np.float64)
+    r = np.mod(s.r - s.low, s.box) + s.low + np.random.default_rng(0).normal(size=s.r.shape) * 0.05
+    a = SO.neighbour_pairs(r, box)
+    b = SO.neighbour_pairs_cells(r, box)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b)) and len(a[0]) > 10 * s.n
+    cen = np.random.default_rng(1).choice(s.n, 57, replace=False)
+    keep = np.isin(a[0], cen)
+    c = SO.neighbour_pairs_cells(r, box, centers=cen)
+    assert all(np.array_equal(x[keep], y) for x, y in zip(a, c))
+
+
+def test_subset_evaluation_equals_full():
+    """Descriptors, site energies and forces of a site subset (the network on the subset's
+    neighbourhood only) equal the whole-system evaluation, bit for bit."""
+    from komd_amd import systems as S
+    s = S.fcc(8, 0.8, 0.8, 24, jitter=0.1)
+    box = np.asarray(s.box, np.float64)
+    r = np.mod(s.r - s.low, s.box) + s.low
+    p = SO.init_params(5, e_mean=0.2, e_std=0.05)
+    i, j, d = SO.neighbour_pairs(r, box)
+    D, _ = SO.descriptors(s.n, i, d)
+    _, E, F = SO.energy_forces(p, s.n, i, j, d)
+    sub = np.unique(np.random.default_rng(2).choice(s.n, 120, replace=False))
+    Ds, Es, Fs = SO.energy_forces_subset(p, r, box, sub)
+    assert np.array_equal(Ds, D[sub]) and np.array_equal(Es, E[sub]) and np.array_equal(Fs, F[sub])
