@@ -56,7 +56,10 @@ enum {
 /* Pair force kinds for komd_forces_pair and komd_select_pair. */
 enum {
     KOMD_PAIR_LJ12_6 = 1,   /* LJ 12-6, Lorentz-Berthelot mixing, same-molecule exclusion  */
-    KOMD_PAIR_TABLE = 2     /* tabulated radial force (the IBI force functor), no exclusion */
+    KOMD_PAIR_TABLE = 2,    /* tabulated radial force (the IBI force functor), no exclusion */
+    KOMD_PAIR_ATM2B = 3     /* LJ 12-6 times 1 - 2 nu rho/(3 eps sig^6): the mean-field pair
+                               form of ATM (potentials/ATM2B.cpp:9-56; rho = molecules per
+                               volume, eps and sigma of the first site); energy scaled alike */
 };
 
 typedef struct {
@@ -170,8 +173,9 @@ komd_status komd_set_pair_table(komd_ctx* ctx, int64_t nodes, const double* x, c
                                 double force_low, double force_high, const double* potential, double potential_low,
                                 double potential_high);
 /* The pair force that komd_run, komd_compute_forces and komd_measure_observables apply:
- * KOMD_PAIR_LJ12_6 (default) or KOMD_PAIR_TABLE (needs komd_set_pair_table), as the IBI
- * plugin replaces the LJ functor in the registry (plugins/IBI/IBI.cpp:51-55). */
+ * KOMD_PAIR_LJ12_6 (default), KOMD_PAIR_TABLE (needs komd_set_pair_table) or
+ * KOMD_PAIR_ATM2B, as the IBI plugin replaces the LJ functor in the registry
+ * (plugins/IBI/IBI.cpp:51-55) and main.cpp registers ATM2B in LJ's place. */
 komd_status komd_select_pair(komd_ctx* ctx, int32_t kind);
 /* Harmonic bond and angle forces (new; replaces the reference's FENE functor slot). */
 komd_status komd_forces_bonded(komd_ctx* ctx);

@@ -23,7 +23,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_STATE", 4: "ERR_OVERFLOW"
           6: "ERR_NOMEM"}
 BIN_COM, BIN_SITE = 0, 1
 ATM_EXACT, ATM_REF, SURROGATE, HYBRID = 1, 2, 3, 4
-PAIR_LJ12_6, PAIR_TABLE = 1, 2
+PAIR_LJ12_6, PAIR_TABLE, PAIR_ATM2B = 1, 2, 3
 SUR_NDESC, SUR_H1, SUR_H2 = 48, 64, 64
 SUR_NPARAM = 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1 + 48 + 48
 

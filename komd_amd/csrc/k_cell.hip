@@ -79,6 +79,7 @@ struct CellArgs {
     // workgroups each (placed last), so the second round holds half-cells only
     int cx, slots;
     float rc2f, band, sig2, eps24, eps4;
+    float c24, c4;  // 24 and 4 times the pair-force scale (mixed sites: e24 = c24 eps_ij)
     double rc2, limit, kick;
 };
 
@@ -248,8 +249,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                         const float s = 0.5f * (pi.x + pj.x);
                         const float e = pi.y * pj.y;
                         sig2 = s * s;
-                        e24 = 24.f * e;
-                        e4 = 4.f * e;
+                        e24 = a.c24 * e;
+                        e4 = a.c4 * e;
                     }
                     const float inv = __builtin_amdgcn_rcpf(r2);
                     const float s2 = sig2 * inv;
@@ -650,7 +651,7 @@ int cell_blocks(komd_ctx* c) {
 }
 int atm_blocks(komd_ctx* c) { return 8 * (((int)((c->n + kAtmWaves - 1) / kAtmWaves) + 7) / 8); }
 
-hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
+hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, double scale) {
     auto& d = c->d;
     CellArgs a{};
     a.spos = d.spos; a.sprm = d.sprm; a.sid = d.sid; a.start = d.start;
@@ -667,8 +668,12 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit) {
     a.rc2f = (float)a.rc2;
     a.band = 4e-6f * a.rc2f;
     a.sig2 = c->sig_u * c->sig_u;
-    a.eps24 = 24.f * c->eps_u;
-    a.eps4 = 4.f * c->eps_u;
+    // LJ's 24 eps and 4 eps with the pair-force scale folded in (1 for LJ 12-6, the
+    // mean-field factor for ATM2B): no extra operation per pair
+    a.eps24 = (float)(24.0 * (double)c->eps_u * scale);
+    a.eps4 = (float)(4.0 * (double)c->eps_u * scale);
+    a.c24 = (float)(24.0 * scale);
+    a.c4 = (float)(4.0 * scale);
     a.limit = c->prm.limit_factor;
     a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
     a.kick = (lj && !energy && c->kick_in == KICK_IN_CELL) ? 0.5 * c->prm.delta_t : 0.0;

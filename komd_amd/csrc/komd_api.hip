@@ -184,6 +184,15 @@ int atm_code(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind == KOMD_3B
 // (the reference-mode ATM filter and the surrogate's per-cell force rows read them)
 int nl_mode(int kind) { return kind == KOMD_3B_ATM_EXACT ? 1 : (kind ? 2 : 0); }
 bool table_pair(const komd_ctx* c) { return c->pair_kind == KOMD_PAIR_TABLE; }
+// ATM2B's pair-force factor 1 - 2 nu rho / (3 eps sig^6) (potentials/ATM2B.cpp:9-20): rho is
+// the molecule number density, eps and sigma those of the first site (one component)
+double atm2b_factor(const komd_ctx* c) {
+    const Grid& g = c->grid;
+    const double rho = (double)c->nmol / (g.size[0] * g.size[1] * g.size[2]);
+    return 1.0 - (2.0 * c->prm.nu * rho) / (3.0 * c->eps0 * std::pow(c->sig0, 6));
+}
+// the LJ pass's force scale of a pair kind
+double pair_scale(const komd_ctx* c, int kind) { return kind == KOMD_PAIR_ATM2B ? atm2b_factor(c) : 1.0; }
 // lists of a step: the three-body kind's, and at least plain lists for the tabulated pair force
 int lists_of(const komd_ctx* c, int kind3b) { return table_pair(c) ? std::max(1, nl_mode(kind3b)) : nl_mode(kind3b); }
 // The kernel that applies the second half-kick of an NVE step: the last force kernel that
@@ -224,7 +233,7 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     // from the site's compacted list (one pair per lane) instead of the pair pass's
     // candidate loop, where a lane's rare hits left most lanes idle
     // (the surrogate's descriptor pass likewise walks every site's list)
-    const bool lj_mv = !table_pair(c) && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+    const bool lj_mv = c->pair_kind == KOMD_PAIR_LJ12_6 && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
     const bool lj_atm = atm && lj_mv, lj_sur = kind3b == KOMD_3B_SURROGATE && lj_mv;
     if (table_pair(c)) {
         // tabulated pair force (the IBI functor in LJ's registry slot, IBI.cpp:51-55) from
@@ -233,7 +242,7 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
         KOMD_CHECK_HIP(launch_table(c, false));
         KOMD_CHECK_HIP(launch_limit(c));
     } else {
-        KOMD_CHECK_HIP(launch_cell(c, !(lj_atm || lj_sur), nl, false, true));
+        KOMD_CHECK_HIP(launch_cell(c, !(lj_atm || lj_sur), nl, false, true, pair_scale(c, c->pair_kind)));
     }
     if (atm) return launch_atm(c, atm, false, nullptr, nullptr, nullptr, lj_atm);
     if (kind3b == KOMD_3B_SURROGATE) {
@@ -403,6 +412,8 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
         if (eps[i] != eps[0] || sig[i] != sig[0]) { c->uniform = false; break; }
     c->eps_u = (float)eps[0];
     c->sig_u = (float)sig[0];
+    c->eps0 = eps[0];
+    c->sig0 = sig[0];
     // SoA split of the reference's interleaved d3 layout
     std::vector<double> buf((size_t)n);
     const size_t nb = (size_t)n * sizeof(double);
@@ -622,8 +633,8 @@ komd_status komd_forces_pair(komd_ctx* c, int32_t kind) {
         HIPCALL(c, launch_table(c, false), "komd_forces_pair");
         return sync_check(c, "komd_forces_pair");
     }
-    if (kind != KOMD_PAIR_LJ12_6) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
-    HIPCALL(c, launch_cell(c, true, 0, false, false), "komd_forces_pair");
+    if (kind != KOMD_PAIR_LJ12_6 && kind != KOMD_PAIR_ATM2B) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
+    HIPCALL(c, launch_cell(c, true, 0, false, false, pair_scale(c, kind)), "komd_forces_pair");
     return sync_check(c, "komd_forces_pair");
 }
 
@@ -650,7 +661,8 @@ komd_status komd_set_pair_table(komd_ctx* c, int64_t nodes, const double* x, con
 
 komd_status komd_select_pair(komd_ctx* c, int32_t kind) {
     if (!c) return KOMD_ERR_ARG;
-    if (kind != KOMD_PAIR_LJ12_6 && kind != KOMD_PAIR_TABLE) return fail(c, KOMD_ERR_ARG, "unknown pair kind");
+    if (kind != KOMD_PAIR_LJ12_6 && kind != KOMD_PAIR_TABLE && kind != KOMD_PAIR_ATM2B)
+        return fail(c, KOMD_ERR_ARG, "unknown pair kind");
     if (kind == KOMD_PAIR_TABLE && !c->tab)
         return fail(c, KOMD_ERR_STATE, "no pair table: call komd_set_pair_table first");
     if (kind != c->pair_kind) drop_graph(c);
@@ -909,7 +921,7 @@ komd_status measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* o
         out->u_lj = vals[0];
         if ((s = add_virial(table_blocks(c)))) return s;
     } else {
-        HIPCALL(c, launch_cell(c, true, nl, true, false), "komd_measure_observables");
+        HIPCALL(c, launch_cell(c, true, nl, true, false, pair_scale(c, c->pair_kind)), "komd_measure_observables");
         if ((s = reduce(d.red, cell_blocks(c), 1))) return s;
         out->u_lj = vals[0];
         if ((s = add_virial(cell_blocks(c)))) return s;

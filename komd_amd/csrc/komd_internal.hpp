@@ -114,6 +114,7 @@ struct komd_ctx {
     bool uniform = true;       // all sites share sigma/epsilon
     bool multisite = false;    // some molecule has more than one site
     float sig_u = 1.f, eps_u = 1.f;
+    double sig0 = 1.0, eps0 = 1.0;  // the first site's sigma and epsilon (ATM2B's one component)
     int64_t nbonds = 0, nangles = 0;
     std::vector<uint32_t> mol_start_h;  // host copy of the molecule ranges
     bool bonded_by_mol = false;         // every bond/angle inside one molecule: per-molecule kernel
@@ -218,7 +219,8 @@ hipError_t rdf_scratch_init(komd_ctx* c);
 // atm: 1 physical, 2 reference; lam/dlam: hybrid weights (nullptr otherwise); esite:
 // per-site energy share out (optional).
 enum { KICK_NONE = 0, KICK_IN_CELL = 1, KICK_IN_ATM = 2, KICK_IN_APPLY = 3, KICK_IN_TABLE = 4 };
-hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit);
+// scale multiplies the LJ force and energy (1: LJ 12-6; the ATM2B factor for KOMD_PAIR_ATM2B)
+hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, double scale = 1.0);
 // lj: the kernel also adds the LJ force of every listed pair (uniform single-site
 // systems without the force clamp; the pair pass then skips LJ)
 hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite,

@@ -72,6 +72,9 @@ bool readFile(const std::string& path, Configuration& config, std::vector<Compon
         else if (var == "hybrid_region") file >> config.hybrid_axis >> config.hybrid_center >> config.hybrid_half >> config.hybrid_blend;
         else if (var == "thermostat") file >> config.thermostat;
         else if (var == "checkpoint_precision") file >> config.checkpoint_precision;
+        else if (var == "enable_3b_approx") file >> config.enable_3b_approx;
+        else if (var == "pair_table") file >> config.pair_table;
+        else if (var == "pair_table_potential") file >> config.pair_table_potential;
         else if (var == "enable_sensor_rdf") file >> config.enable_sensor_rdf;
         else if (var == "enable_sensor_pres") file >> config.enable_sensor_pres;
         else if (var == "sensor_rdf_max") file >> config.sensor_rdf_max;
@@ -195,6 +198,83 @@ std::array<int64_t, 3> LinkedCells::gridDims() const {
 }
 
 void LJ12_6::handlePairList() { check(p_container.ctx(), komd_forces_pair(p_container.ctx(), KOMD_PAIR_LJ12_6), "LJ12_6"); }
+void ATM2B::handlePairList() { check(p_container.ctx(), komd_forces_pair(p_container.ctx(), KOMD_PAIR_ATM2B), "ATM2B"); }
+
+double FunctionPL::evaluateAt(double x) const {
+    const size_t n = m_x.size();
+    if (n == 0) throw std::runtime_error("FunctionPL: empty function");
+    if (x > m_x[n - 1]) return m_high;
+    if (x < m_x[0]) return m_low;
+    size_t i = 0;  // findUpperNode: the first node >= x
+    while (x > m_x[i]) {
+        i++;
+        if (i == n) break;
+    }
+    if (i == 0) return m_y[0];
+    const double ya = m_y[i - 1], yb = m_y[i], xa = m_x[i - 1], xb = m_x[i];
+    return ya + (yb - ya) / (xb - xa) * (x - xa);
+}
+
+void FunctionPL::derivative(FunctionPL& target) const {
+    const size_t n = m_y.size();
+    if (n < 2) throw std::runtime_error("FunctionPL::derivative: fewer than two nodes");
+    target.m_x = m_x;
+    target.m_y.assign(n, 0.0);
+    target.m_y[0] = (m_y[1] - m_y[0]) / (m_x[1] - m_x[0]);
+    target.m_y[n - 1] = (m_y[n - 1] - m_y[n - 2]) / (m_x[n - 1] - m_x[n - 2]);
+    for (size_t k = 1; k + 1 < n; k++) target.m_y[k] = (m_y[k + 1] - m_y[k - 1]) / (m_x[k + 1] - m_x[k - 1]);
+}
+
+void FunctionPL::read(const std::string& path) {
+    std::ifstream file{path};
+    if (!file) throw std::runtime_error("Failed to open file: " + path);
+    std::vector<double> x, y;
+    double a, b;
+    while (file >> a >> b) {
+        x.push_back(a);
+        y.push_back(b);
+    }
+    m_x = x;
+    m_y = y;
+}
+
+void FunctionPL::write(const std::string& path) const {
+    std::ofstream file{path};
+    if (!file) throw std::runtime_error("Failed to open file: " + path);
+    file.precision(17);
+    for (size_t k = 0; k < m_x.size(); k++) {
+        if (k) file << "\n";
+        file << m_x[k] << "\t" << m_y[k];
+    }
+}
+
+void IBI_ForceFunctor::upload() {
+    const auto& x = m_force.getXValues();
+    const auto& y = m_force.getYValues();
+    // the potential counts only on the force's nodes
+    const bool pot = m_potential.getXValues() == x && m_potential.getYValues().size() == x.size();
+    std::vector<double> now(x);
+    now.insert(now.end(), y.begin(), y.end());
+    now.push_back(m_force.getLowerDefault());
+    now.push_back(m_force.getUpperDefault());
+    if (pot) {
+        now.insert(now.end(), m_potential.getYValues().begin(), m_potential.getYValues().end());
+        now.push_back(m_potential.getLowerDefault());
+        now.push_back(m_potential.getUpperDefault());
+    }
+    if (now == m_sent) return;
+    check(p_container.ctx(),
+          komd_set_pair_table(p_container.ctx(), (int64_t)x.size(), x.data(), y.data(), m_force.getLowerDefault(),
+                              m_force.getUpperDefault(), pot ? m_potential.getYValues().data() : nullptr,
+                              m_potential.getLowerDefault(), m_potential.getUpperDefault()),
+          "IBI_ForceFunctor");
+    m_sent = std::move(now);
+}
+
+void IBI_ForceFunctor::handlePairList() {
+    upload();
+    check(p_container.ctx(), komd_forces_pair(p_container.ctx(), KOMD_PAIR_TABLE), "IBI_ForceFunctor");
+}
 void Limit::handlePairList() { check(p_container.ctx(), komd_forces_limit(p_container.ctx()), "Limit"); }
 void HarmonicBonds::handlePairList() { check(p_container.ctx(), komd_forces_bonded(p_container.ctx()), "HarmonicBonds"); }
 void ATM::handleTripleList() { check(p_container.ctx(), komd_forces_3b(p_container.ctx(), m_kind), "ATM"); }
@@ -327,7 +407,22 @@ std::unique_ptr<Registry> makeRegistry(const Configuration& config) {
     lc->init();
     reg->container = std::move(lc);
     MoleculeContainer& c = *reg->container;
-    reg->forceFunctors.push_back(std::make_shared<LJ12_6>(c));
+    // the pair functor (main.cpp:56-57): LJ 12-6, ATM2B with enable_3b_approx, or the
+    // tabulated force of a pair_table file (the IBI functor the plugin registers in LJ's
+    // place, IBI.cpp:51-55, with the plugin's defaults 1e6 below and 0 above the nodes)
+    if (!config.pair_table.empty()) {
+        auto ibi = std::make_shared<IBI_ForceFunctor>(c);
+        ibi->getForceFunction().read(config.pair_table);
+        if (!config.pair_table_potential.empty()) ibi->getPotentialFunction().read(config.pair_table_potential);
+        ibi->upload();
+        reg->forceFunctors.push_back(ibi);
+        check(c.ctx(), komd_select_pair(c.ctx(), KOMD_PAIR_TABLE), "pair_table");
+    } else if (config.enable_3b_approx) {
+        reg->forceFunctors.push_back(std::make_shared<ATM2B>(c));
+        check(c.ctx(), komd_select_pair(c.ctx(), KOMD_PAIR_ATM2B), "enable_3b_approx");
+    } else {
+        reg->forceFunctors.push_back(std::make_shared<LJ12_6>(c));
+    }
     reg->forceFunctors.push_back(std::make_shared<Limit>(c));
     if (config.enable_3b) {
         c.enable3B();
@@ -397,6 +492,16 @@ void Simulation::runDevice(Registry& reg, uint64_t steps) {
         if (auto s = std::dynamic_pointer_cast<ATMSurrogate>(reg.forceFunctors3b[0])) kind = s->kind();
     }
     const double t0 = reg.thermostats.empty() ? 0.0 : reg.config.temperature;
+    int pair = KOMD_PAIR_LJ12_6;
+    if (!reg.forceFunctors.empty()) {
+        if (auto ibi = std::dynamic_pointer_cast<IBI_ForceFunctor>(reg.forceFunctors[0])) {
+            ibi->upload();
+            pair = KOMD_PAIR_TABLE;
+        } else if (std::dynamic_pointer_cast<ATM2B>(reg.forceFunctors[0])) {
+            pair = KOMD_PAIR_ATM2B;
+        }
+    }
+    check(c.ctx(), komd_select_pair(c.ctx(), pair), "Simulation::runDevice");
     check(c.ctx(), komd_compute_forces(c.ctx(), kind, 0), "Simulation::runDevice");
     if (t0 > 0) {
         reg.temperatureSensor->measure();

@@ -12,6 +12,8 @@
 //   Sensor / RDFSensor / PressureSensor  sensors/Sensor.h, RDFSensor.cpp:13-57, PressureSensor.cpp:21-70
 //   Thermostat / VelocityScaling     thermostats/VelocityScaling.cpp:12-25
 //   Simulation::run                  Simulation.cpp:10-80
+//   ATM2B                            potentials/ATM2B.cpp:9-56
+//   FunctionPL / IBI_ForceFunctor    plugins/IBI/IBI_Math.h:13-78, IBI_Math.cpp, IBI_ForceFunctor.cpp:10-42
 //   FileInput::readFile              IO/FileInput.cpp:11-121
 //   CheckpointIO                     IO/CheckpointIO.cpp:13-76
 #pragma once
@@ -52,6 +54,7 @@ struct Configuration {
     bool storeCheckpoint = false;
     bool enable_3b = false;
     bool enable_3b_direct = true;
+    bool enable_3b_approx = false;       // ATM2B in LJ's functor slot (main.cpp:56-57)
     std::vector<std::pair<std::string, d3>> checkpoint_files;
     // GPU path options (no reference counterpart)
     std::string atm_mode = "reference";  // "reference": ATM_NOLIST semantics; "exact": physical, all triplets
@@ -62,6 +65,8 @@ struct Configuration {
     double hybrid_center = 0, hybrid_half = 0, hybrid_blend = 2.0;
     bool thermostat = true;              // velocity rescaling every step (main.cpp:93)
     int checkpoint_precision = 17;       // digits of the .ps writer
+    std::string pair_table;              // tabulated pair force (FunctionPL file) in LJ's slot
+    std::string pair_table_potential;    // its potential on the same nodes (energies only)
     // sensors (defaults.h:30-46)
     bool enable_sensor_rdf = false;
     bool enable_sensor_pres = false;
@@ -154,6 +159,64 @@ public:
 
 protected:
     void handlePairList() override;
+};
+
+// ATM2B (potentials/ATM2B.cpp:9-56): LJ 12-6 times 1 - 2 nu rho / (3 eps sig^6), registered
+// in LJ's place when the deck sets enable_3b_approx (main.cpp:56-57).
+class ATM2B : public ForceFunctor {
+public:
+    using ForceFunctor::ForceFunctor;
+
+protected:
+    void handlePairList() override;
+};
+
+// FunctionPL (plugins/IBI/IBI_Math.h:13-78, IBI_Math.cpp): piecewise-linear function of
+// ascending nodes x with values y, the lower default below the first node and the upper
+// default above the last; read / write use the reference's two-column text format.
+class FunctionPL {
+public:
+    explicit FunctionPL(size_t buffer_size = 0, double def_low = 0.0, double def_high = 0.0)
+        : m_x(buffer_size, 0.0), m_y(buffer_size, 0.0), m_low(def_low), m_high(def_high) {}
+    void setXValues(const std::vector<double>& x) { m_x = x; }
+    void setYValues(const std::vector<double>& y) { m_y = y; }
+    std::vector<double>& getXValues() { return m_x; }
+    std::vector<double>& getYValues() { return m_y; }
+    const std::vector<double>& getXValues() const { return m_x; }
+    const std::vector<double>& getYValues() const { return m_y; }
+    double getLowerDefault() const { return m_low; }
+    double getUpperDefault() const { return m_high; }
+    void setLowerDefault(double v) { m_low = v; }
+    void setUpperDefault(double v) { m_high = v; }
+    // evaluateAt (IBI_Math.h:30-41); at x equal to the first node: the first value
+    double evaluateAt(double x) const;
+    // forward / backward differences at the ends, central ones inside (IBI_Math.cpp:22-35)
+    void derivative(FunctionPL& target) const;
+    void read(const std::string& path);
+    void write(const std::string& path) const;
+
+private:
+    std::vector<double> m_x, m_y;
+    double m_low, m_high;
+};
+
+// IBI_ForceFunctor<IBI_Default> (plugins/IBI/IBI_ForceFunctor.cpp:10-42): the tabulated pair
+// force.  Its tables may be edited in place between steps (as the IBI plugin does), so each
+// call copies them to the device when they differ from the last copy.
+class IBI_ForceFunctor : public ForceFunctor {
+public:
+    explicit IBI_ForceFunctor(MoleculeContainer& container)
+        : ForceFunctor(container), m_force(0, 1e+6, 0.0), m_potential(0, 1e+6, 0.0) {}
+    FunctionPL& getForceFunction() { return m_force; }
+    FunctionPL& getPotentialFunction() { return m_potential; }
+    void upload();
+
+protected:
+    void handlePairList() override;
+
+private:
+    FunctionPL m_force, m_potential;
+    std::vector<double> m_sent;  // the tables and defaults of the last copy
 };
 
 class Limit : public ForceFunctor {

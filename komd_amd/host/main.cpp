@@ -43,6 +43,8 @@ int main(int argc, char** argv) {
             return 1;
         }
         if (steps < 0) steps = (long long)config.timesteps;
+        // the pair functor the registry puts in LJ's slot (main.cpp:56-57)
+        const char* pair = !config.pair_table.empty() ? "table" : (config.enable_3b_approx ? "atm2b" : "lj12_6");
         if (check_only) {
             Sites sites;
             CheckpointIO::loadCheckpoint(config, sites);
@@ -54,11 +56,11 @@ int main(int argc, char** argv) {
                 "{\"sites\": %zu, \"molecules\": %llu, \"components\": %zu, \"cutoff\": %.17g, \"cell_size\": %.17g, "
                 "\"delta_t\": %.17g, \"temperature\": %.17g, \"nu_internal\": %.17g, \"limit\": %.17g, "
                 "\"domain_high\": [%.17g, %.17g, %.17g], \"enable_3b\": %d, \"atm_mode\": \"%s\", \"timesteps\": %lld, "
-                "\"first_site\": [%.17g, %.17g, %.17g]}\n",
+                "\"first_site\": [%.17g, %.17g, %.17g], \"pair\": \"%s\"}\n",
                 sites.size(), (unsigned long long)mols, comps.size(), config.cutoff, config.cell_size, config.delta_t,
                 config.temperature, config.energy_3b * 1e9 * kConvJEi, config.limit_factor, config.domainHigh[0],
                 config.domainHigh[1], config.domainHigh[2], (int)config.enable_3b, config.atm_mode.c_str(), steps,
-                sites.size() ? sites.r[0] : 0.0, sites.size() ? sites.r[1] : 0.0, sites.size() ? sites.r[2] : 0.0);
+                sites.size() ? sites.r[0] : 0.0, sites.size() ? sites.r[1] : 0.0, sites.size() ? sites.r[2] : 0.0, pair);
             return 0;
         }
         auto reg = makeRegistry(config);
@@ -78,11 +80,11 @@ int main(int argc, char** argv) {
         std::printf(
             "{\"sites\": %zu, \"molecules\": %llu, \"grid\": [%lld, %lld, %lld], \"steps\": %llu, \"loop\": \"%s\", "
             "\"temperature\": %.17g, \"kinetic\": %.17g, \"u_lj\": %.17g, \"u_3b\": %.17g, \"virial\": %.17g, "
-            "\"pairs\": %lld, \"triplets\": %lld, \"sensors\": %zu}\n",
+            "\"pairs\": %lld, \"triplets\": %lld, \"sensors\": %zu, \"pair\": \"%s\"}\n",
             reg->container->sites().size(), (unsigned long long)reg->container->getNumMolecules(), (long long)dims[0],
             (long long)dims[1], (long long)dims[2], (unsigned long long)sim.simstep(),
             device_loop ? "device" : "reference", o.temperature, o.kinetic, o.u_lj, o.u_3b, o.virial,
-            (long long)o.pairs, (long long)o.triplets, reg->sensors.size());
+            (long long)o.pairs, (long long)o.triplets, reg->sensors.size(), pair);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "komd_md: %s\n", e.what());
         return 1;

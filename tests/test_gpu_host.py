@@ -154,3 +154,42 @@ def test_sensor_files(orc, tmp_path):
         o = c.observables(K.ATM_REF)
     ref = (o["kinetic_tensor"] + o["virial_tensor"]) / vol * 1.66053906892e7
     assert np.allclose(vals, ref, rtol=2e-5, atol=2e-5 * np.abs(ref).max())
+
+
+def _lj_table_files(tmp_path):
+    x = np.linspace(0.7, 2.5, 4001)
+    f = 24.0 * (2.0 * x ** -13 - x ** -7)
+    u = 4.0 * (x ** -12 - x ** -6)
+    ff, uf = tmp_path / "force.txt", tmp_path / "pot.txt"
+    np.savetxt(ff, np.c_[x, f], fmt="%.17g", delimiter="\t")
+    np.savetxt(uf, np.c_[x, u], fmt="%.17g", delimiter="\t")
+    return str(ff), str(uf)
+
+
+@pytest.mark.parametrize("pair", ["atm2b", "table"])
+def test_pair_functor_decks(tmp_path, pair):
+    """Decks that put ATM2B (enable_3b_approx, main.cpp:56-57) or a tabulated force
+    (pair_table: the IBI functor, IBI.cpp:51-55) in LJ's slot: the energy before the first
+    step is LJ's times the ATM2B factor, or LJ's to interpolation error for an LJ-sampled
+    table; the device loop follows the per-functor loop over 12 steps."""
+    s = S.fcc(6, 0.8, 0.8, 75, jitter=0.05)
+    if pair == "atm2b":
+        extra = "enable_3b 0\nenable_3b_approx 1\n"
+    else:
+        ff, uf = _lj_table_files(tmp_path)
+        extra = f"enable_3b 0\npair_table {ff}\npair_table_potential {uf}\n"
+    base = run_md(deck_for(tmp_path, s, 0, extra="enable_3b 0\n"), "--steps", "0")
+    d = deck_for(tmp_path, s, 12, extra=extra)
+    i0 = run_md(d, "--steps", "0")
+    assert i0["pair"] == pair and base["pair"] == "lj12_6"
+    if pair == "atm2b":
+        fac = 1.0 - 2.0 * NU * (s.n / float(np.prod(s.box))) / 3.0
+        assert i0["u_lj"] == pytest.approx(fac * base["u_lj"], rel=1e-6)
+    else:
+        assert i0["u_lj"] == pytest.approx(base["u_lj"], rel=1e-4)
+    a, b = str(tmp_path / "a.ps"), str(tmp_path / "b.ps")
+    run_md(d, "--write", a)
+    run_md(d, "--device-loop", "--write", b)
+    ra, rb = np.loadtxt(a), np.loadtxt(b)
+    assert np.abs(ra[:, 1:4] - rb[:, 1:4]).max() < 1e-9
+    assert np.abs(ra[:, 4:7] - rb[:, 4:7]).max() < 1e-7

@@ -234,14 +234,21 @@ def test_judge_and_fallback():
 
 def test_trained_surrogate_gain():
     """What the trained network gains over the methods without it: three-body force error
-    against the exact ATM on an equilibrated argon state (rho 0.8, T* 0.8), for the
-    committed weights, the mean-field ATM2B scaling (ATM2B.cpp:9-20) and LJ alone."""
+    against the exact ATM on a thermostatted argon state (rho 0.8, T* 0.8, 4,000 atoms) for
+    the committed weights, and for comparators that cost one pair pass: the mean-field
+    ATM2B pair force (ATM2B.cpp:9-56, KOMD_PAIR_ATM2B), an IBI table fitted to the exact
+    path's g(r) (the IBI plugin's iteration, IBI.cpp:26-171, on KOMD_PAIR_TABLE) and LJ
+    alone.  The comparators' three-body estimate is their pair force minus LJ's."""
+    from komd_amd import ibi
     from komd_amd.train import DEFAULT_WEIGHTS
     s = S.fcc(10, 0.8, 0.8, 2024)
+    vol = float(np.prod(s.box))
     with K.Context(s, nu=NU, dt=0.005) as c:
         c.surrogate_set_params(np.load(DEFAULT_WEIGHTS))
         c.compute_forces(K.ATM_EXACT)
-        c.run(400, kind3b=K.ATM_EXACT)
+        c.run(400, kind3b=K.ATM_EXACT, t0=0.8)
+        g_target = ibi.measure_g(c, s.n, vol, 2.5, 100, 50, 10, K.ATM_EXACT, 0.8)
+        r, v, _ = c.get_state()
         rel, mx = c.surrogate_judge()
         c.update_container()
         c.forces_3b(K.ATM_EXACT)
@@ -249,11 +256,28 @@ def test_trained_surrogate_gain():
         c.update_container()
         c.forces_pair()
         f_lj = c.forces()
-    scale = np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1)))
-    err_2b = np.sqrt(np.mean(np.sum((-(2 * NU * 0.8 / 3) * f_lj - f_atm) ** 2, axis=1))) / scale
-    print(f"three-body force error: surrogate {rel:.4f} (max {mx:.3f}), ATM2B {err_2b:.4f}, LJ only 1.0")
+        c.update_container()
+        c.forces_pair(K.PAIR_ATM2B)
+        f_2b = c.forces()
+    s2 = s.copy()
+    s2.r, s2.v = r, v
+    with K.Context(s2, dt=0.005) as d:
+        _, _, _, hist = ibi.run(d, g_target, s.n, vol, 2.5, 0.8, iterations=8)
+        d.set_state(r=r, v=v)
+        d.update_container()
+        d.forces_pair(K.PAIR_TABLE)
+        f_ibi = d.forces()
+
+    def rms(a):
+        return float(np.sqrt(np.mean(np.sum(a ** 2, axis=1))))
+    scale = rms(f_atm)
+    err_2b = rms(f_2b - f_lj - f_atm) / scale
+    err_ibi = rms(f_ibi - f_lj - f_atm) / scale
+    print(f"three-body force error: surrogate {rel:.4f} (max {mx:.3f}), ATM2B {err_2b:.4f}, IBI {err_ibi:.4f} "
+          f"(g RMS error {hist[0]:.4f} -> {hist[-1]:.4f}), LJ only 1.0")
     assert rel < 0.06
-    assert rel < 0.25 * err_2b
+    assert rel < 0.25 * err_2b and rel < 0.25 * err_ibi
+    assert hist[-1] < hist[0]  # the iteration moves g(r) toward the target
 
 
 @pytest.mark.parametrize("half,blend", [(1.5, 2.0), (0.8, 1.0)])

@@ -185,3 +185,40 @@ def test_table_argument_checks():
             c.set_pair_table([1.0, 2.0], [0.0, np.nan])
         with pytest.raises(K.KomdError, match="2 to 2"):
             c.set_pair_table([1.0], [0.0])
+
+
+def test_atm2b_forces_and_run(orc):
+    """KOMD_PAIR_ATM2B (potentials/ATM2B.cpp:9-56): the LJ force and energy times
+    1 - 2 nu rho / (3 eps sig^6) (rho = sites per volume here), against the oracle's LJ
+    times that factor; the graphed run with ATM2B selected equals the per-functor loop."""
+    from oracle import surrogate as SO
+    s = S.fcc(10, 0.8, 0.8, 407, jitter=0.1)
+    nu = 0.073
+    fac = SO.atm2b_scale(nu, s.n / float(np.prod(s.box)))
+    r, b = orc.update_container(s)
+    f_lj, u_lj, _ = orc.lj_c08(s, r, b, energy=True, nthreads=8)
+    with K.Context(s, nu=nu) as c:
+        c.update_container()
+        c.forces_pair(K.PAIR_ATM2B)
+        f = c.forces()
+        c.select_pair(K.PAIR_ATM2B)
+        o = c.observables(0)
+    mx, rms = rel_err(f, fac * f_lj)
+    print(f"ATM2B factor {fac:.6f}: max {mx:.3e} rms {rms:.3e}")
+    assert abs(fac - (1 - 2 * nu * 0.8 / 3)) < 1e-12
+    assert mx < 2e-4 and rms < 5e-6
+    assert abs(o["u_lj"] - fac * u_lj) < 1e-5 * abs(u_lj)
+    small = S.fcc(6, 0.8, 0.8, 408, jitter=0.05)
+    with K.Context(small, nu=nu) as a, K.Context(small, nu=nu) as b2:
+        for c in (a, b2):
+            c.select_pair(K.PAIR_ATM2B)
+            c.compute_forces(0)
+        a.run(12)
+        for _ in range(12):
+            b2.integrate0()
+            b2.update_container()
+            b2.forces_pair(K.PAIR_ATM2B)
+            b2.integrate1()
+        ra, va, _ = a.get_state()
+        rb, vb, _ = b2.get_state()
+    assert np.array_equal(ra, rb) and np.array_equal(va, vb)

@@ -93,3 +93,15 @@ def test_deck_errors(tmp_path, mutation, message):
     d = deck(tmp_path, mutation(IBI_DECK.format(ps=IBI_PS)))
     res = run(d, "--check", check=False)
     assert res.returncode == 1 and message in res.stderr
+
+
+@pytest.mark.parametrize("extra,pair", [("", "lj12_6"), ("enable_3b_approx 1\n", "atm2b"),
+                                        ("pair_table {t}\n", "table")])
+def test_pair_functor_keys(tmp_path, extra, pair):
+    """The reference's enable_3b_approx key (FileInput.cpp:42) puts ATM2B in LJ's slot
+    (main.cpp:56-57); pair_table puts the tabulated force there."""
+    t = tmp_path / "force.txt"
+    t.write_text("0.9\t10\n1.5\t2\n2.5\t0")
+    d = deck(tmp_path, IBI_DECK.format(ps=IBI_PS) + extra.format(t=t))
+    out = json.loads(run(d, "--check").stdout)
+    assert out["pair"] == pair
