@@ -8,6 +8,10 @@ order (oracle: vv_step0 -> update_container -> lj_c08 + atm_complete -> vv_step1
 Writes E_total / N every 100 steps and the slope of a linear fit against time.
 
   python tests/golden/make_drift_fp64.py [--cells 10] [--steps 10000] [--threads 8] [--out FILE]
+                                         [--state FILE]
+
+--state keeps the trajectory state (r, v, f and the energies so far) every 1,000 steps
+and resumes from it when the file exists, so an interrupted hour-long run continues.
 """
 import argparse
 import json
@@ -32,6 +36,7 @@ def main():
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--cells", type=int, default=10)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--state", default=None)
     a = ap.parse_args()
     O.build()
     s = S.fcc(a.cells, 0.8, 0.8, 1003)
@@ -48,8 +53,15 @@ def main():
     r, v = s.r.copy(), s.v.copy()
     r, f, U = forces(r)
     steps, e = [0], [(0.5 * float(np.sum(s.mass[:, None] * v * v)) + U) / n]
+    first = 1
+    if a.state and os.path.exists(a.state):
+        z = np.load(a.state)
+        r, v, f = z["r"], z["v"], z["f"]
+        steps, e = [int(x) for x in z["steps"]], [float(x) for x in z["e"]]
+        first = steps[-1] + 1
+        print(f"resumed at step {steps[-1]}", flush=True)
     t0 = time.time()
-    for k in range(1, a.steps + 1):
+    for k in range(first, a.steps + 1):
         r, v = O.vv_step0(DT, r, v, f, s.mass)
         r, f, U = forces(r)
         v = O.vv_step1(DT, v, f, s.mass)
@@ -58,6 +70,9 @@ def main():
             e.append((0.5 * float(np.sum(s.mass[:, None] * v * v)) + U) / n)
             if k % 1000 == 0:
                 print(f"step {k}  E/N {e[-1]:.10f}  {time.time() - t0:.0f}s", flush=True)
+                if a.state:
+                    np.savez(a.state + ".tmp.npz", r=r, v=v, f=f, steps=np.array(steps), e=np.array(e))
+                    os.replace(a.state + ".tmp.npz", a.state)
     t = np.array(steps) * DT
     slope = float(np.polyfit(t, np.array(e), 1)[0])
     out = {"system": f"{n:,} argon atoms, FCC {a.cells}^3 x 4, rho 0.8, T* 0.8 (seed 1003), LJ rc 2.5 + ATM nu 0.073 "
