@@ -81,7 +81,10 @@ def parse():
                     help="comma list of system sizes for the learned leg ('' to skip)")
     ap.add_argument("--no-hybrid", action="store_true", help="skip the 1,048,576-atom hybrid slab leg")
     ap.add_argument("--no-other", action="store_true", help="skip the LJ-4000 and chains-262144 legs")
-    ap.add_argument("--hybrid-steps", type=int, default=100)
+    ap.add_argument("--no-ibi", action="store_true", help="skip the IBI table fit of the learned leg")
+    ap.add_argument("--hybrid-steps", type=int, default=2000)
+    ap.add_argument("--drift-seeds", type=int, default=5,
+                    help="seeds of the exact / surrogate drift comparison at 32,000 atoms (0: skip)")
     ap.add_argument("--structure-steps", type=int, default=1000,
                     help="NVE steps of the g(r) / pressure comparison, surrogate against exact (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -144,7 +147,10 @@ def learned_leg(n_atoms, args):
         ctx.compute_forces(K.ATM_EXACT)
         ctx.run(args.equil_steps, kind3b=K.ATM_EXACT)
         r0, v0, _ = ctx.get_state()
-        # three-body force error of the surrogate, of ATM2B and of LJ alone against the exact ATM
+        # three-body force error of the surrogate and of the comparators without it against the
+        # exact ATM: ATM2B (KOMD_PAIR_ATM2B), an IBI table fitted to the exact path's g(r)
+        # (KOMD_PAIR_TABLE, 32,000 atoms) and LJ alone; a comparator's three-body estimate is
+        # its pair force minus LJ's
         rel, mx = ctx.surrogate_judge()
         ctx.update_container()
         ctx.forces_3b(K.ATM_EXACT)
@@ -152,13 +158,55 @@ def learned_leg(n_atoms, args):
         ctx.update_container()
         ctx.forces_pair()
         f_lj = ctx.forces()
-        atm2b = (1.0 - 2.0 * NU * RHO / 3.0) - 1.0   # ATM2B.cpp:9-20: LJ scaled by 1 - 2 nu rho / (3 eps sig^6)
+        ctx.update_container()
+        ctx.forces_pair(K.PAIR_ATM2B)
+        f_2b = ctx.forces()
         scale = float(np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1))))
-        err_2b = float(np.sqrt(np.mean(np.sum((atm2b * f_lj - f_atm) ** 2, axis=1)))) / scale
+
+        def err3(f_pair):
+            return float(np.sqrt(np.mean(np.sum((f_pair - f_lj - f_atm) ** 2, axis=1)))) / scale
         out["three_body_force_error"] = {
-            "surrogate": {"rms": rel, "max": mx}, "atm2b_mean_field": {"rms": err_2b}, "lj_only": {"rms": 1.0},
+            "surrogate": {"rms": rel, "max": mx}, "atm2b_mean_field": {"rms": err3(f_2b)}, "lj_only": {"rms": 1.0},
             "reference": "exact ATM (FP32 arithmetic, FP64 accumulation; 1.6e-6 RMS from the FP64 oracle) "
                          f"after {args.equil_steps} NVE steps from the lattice"}
+        ibi_table = None
+        if n_atoms == 32000 and not args.no_ibi:
+            from komd_amd import ibi
+            vol = float(np.prod(sysm.box))
+            kT = ctx.observables(K.ATM_EXACT)["temperature"]
+            ctx.compute_forces(K.ATM_EXACT)
+            g_t = ibi.measure_g(ctx, n, vol, 2.5, 100, 50, 10, K.ATM_EXACT, 0.0)   # NVE samples of the exact path
+            ctx.set_state(r0, v0)
+            t_ibi = time.perf_counter()
+            x_t, f_t, u_t, hist = ibi.run(ctx, g_t, n, vol, 2.5, kT, iterations=8)
+            t_ibi = time.perf_counter() - t_ibi
+            ctx.set_state(r0, v0)
+            ctx.update_container()
+            ctx.forces_pair(K.PAIR_TABLE)
+            out["three_body_force_error"]["ibi_table"] = {
+                "rms": err3(ctx.forces()), "g_rms_error_first_last": [hist[0], hist[-1]], "iterations": len(hist),
+                "fit_s": round(t_ibi, 2), "kT": kT,
+                "note": "IBI (plugins/IBI/IBI.cpp) on the GPU table force: 100 nodes to rc, target g(r) from "
+                        "50 NVE samples of the exact path, 8 iterations of 200 + 500 thermostatted steps"}
+            ibi_table = (x_t, f_t, u_t)
+            ctx.select_pair(K.PAIR_LJ12_6)
+        # the comparators cost one pair pass: their own throughput on the same state
+        comp = {}
+        for name, pair in (("lj_only", K.PAIR_LJ12_6), ("atm2b", K.PAIR_ATM2B), ("ibi_table", K.PAIR_TABLE)):
+            if pair == K.PAIR_TABLE and ibi_table is None:
+                continue
+            if pair == K.PAIR_TABLE:
+                ctx.set_pair_table(ibi_table[0], ibi_table[1], 1e6, 0.0, ibi_table[2], 1e6, 0.0)
+            ctx.select_pair(pair)
+            ctx.set_state(r0, v0)
+            ctx.compute_forces(0)
+            ctx.run(args.warmup, kind3b=0)
+            t0 = time.perf_counter()
+            ctx.run(args.steps, kind3b=0)
+            dt = time.perf_counter() - t0
+            comp[name] = {"value": round(n * args.steps / dt, 1), "ms_per_step": round(dt / args.steps * 1e3, 4)}
+        ctx.select_pair(K.PAIR_LJ12_6)
+        out["comparator_throughput"] = comp
         legs = {}
         for kind, name in ((K.ATM_EXACT, "exact"), (K.SURROGATE, "surrogate")):
             ctx.set_state(r0, v0)
@@ -232,8 +280,13 @@ def learned_leg(n_atoms, args):
 
 
 def hybrid_leg(args):
-    """Config-5 slab: 1,048,576 argon atoms (simple cubic 256 x 64 x 64, rho 0.8, 4:1:1 box);
-    exact ATM in the central 20 % of x, cos^2 blend over 2 sigma, surrogate elsewhere."""
+    """Config 5: 1,048,576 argon atoms (simple cubic 256 x 64 x 64, rho 0.8, 4:1:1 box); exact
+    ATM in the central 20 % of x, cos^2 blend over 2 sigma, surrogate elsewhere.  After a
+    200-step thermostatted melt, hybrid_steps NVE steps in segments of hybrid_steps / 10;
+    after each segment: three-body force error of the hybrid against the exact ATM
+    everywhere, per region, and the temperature of each region; the total (blended,
+    conservative) energy gives the drift.  The same run with the exact ATM everywhere is
+    the reference.  Throughput counts the timed segments only."""
     import komd_amd as K
     from komd_amd import systems as S
     sysm = S.config_slab_1048576()
@@ -241,53 +294,85 @@ def hybrid_leg(args):
     Lx = float(sysm.box[0])
     center, half, blend = float(sysm.low[0]) + 0.5 * Lx, 0.1 * Lx, 2.0
     out = {"n_atoms": n, "box": [float(x) for x in sysm.box], "exact_slab": [center - half, center + half],
-           "blend_width": blend}
+           "blend_width": blend, "steps": args.hybrid_steps}
+    nseg = 10
+    seg = max(1, args.hybrid_steps // nseg)
+
+    def region_masks(r):
+        d = r[:, 0] - center
+        d -= Lx * np.rint(d / Lx)
+        ad = np.abs(d)
+        return {"exact": ad <= half, "blend": (ad > half) & (ad < half + blend), "surrogate": ad >= half + blend}
+
     with K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU) as ctx:
         ctx.surrogate_set_params(np.load(WEIGHTS))
         ctx.set_hybrid(0, center, half, blend)
         ctx.compute_forces(K.HYBRID)
         ctx.run(200, kind3b=K.HYBRID, t0=0.8)   # melt the lattice
         r0, v0, _ = ctx.get_state()
-        # per-region three-body force error against the exact ATM everywhere
-        ctx.update_container()
-        ctx.forces_3b(K.ATM_EXACT)
-        f_atm = ctx.forces()
-        ctx.update_container()
-        ctx.forces_3b(K.HYBRID)
-        f_hyb = ctx.forces()
-        rw, _, _ = ctx.get_state()
-        d = rw[:, 0] - center
-        d -= Lx * np.rint(d / Lx)
-        ad = np.abs(d)
-        regions = {"exact": ad <= half, "blend": (ad > half) & (ad < half + blend), "surrogate": ad >= half + blend}
-        scale = float(np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1))))
-        err = {}
-        for name, m in regions.items():
-            e = np.sqrt(np.mean(np.sum((f_hyb[m] - f_atm[m]) ** 2, axis=1))) / scale if m.any() else None
-            err[name] = {"sites": int(m.sum()), "three_body_rms_error": e}
-        out["regions"] = err
         legs = {}
-        for kind, name in ((K.ATM_EXACT, "exact_everywhere"), (K.HYBRID, "hybrid")):
+        for kind, name in ((K.HYBRID, "hybrid"), (K.ATM_EXACT, "exact_everywhere")):
             ctx.set_state(r0, v0)
             ctx.compute_forces(kind)
             ctx.run(5, kind3b=kind)
-            t0 = time.perf_counter()
-            ctx.run(args.hybrid_steps, kind3b=kind)
-            dt = time.perf_counter() - t0
+            ctx.compute_forces(kind)
+            track = {"step": [], "energy_per_atom": [], "temperature": {k: [] for k in ("exact", "blend", "surrogate")},
+                     "three_body_rms_error": {k: [] for k in ("exact", "blend", "surrogate")}}
+            timed = 0.0
+            for k in range(nseg + 1):
+                o = ctx.observables(kind)
+                track["step"].append(k * seg)
+                track["energy_per_atom"].append((o["kinetic"] + o["u_lj"] + o["u_3b"]) / n)
+                r, v, f_run = ctx.get_state()
+                m = region_masks(r)
+                for reg, mk in m.items():
+                    track["temperature"][reg].append(float(np.sum(v[mk] ** 2) / (3.0 * mk.sum())))
+                if kind == K.HYBRID:
+                    # per-region three-body error of the state reached (forces restored after)
+                    ctx.update_container()
+                    ctx.forces_3b(K.ATM_EXACT)
+                    f_atm = ctx.forces()
+                    ctx.update_container()
+                    ctx.forces_3b(K.HYBRID)
+                    f_hyb = ctx.forces()
+                    ctx.set_state(f=f_run)
+                    scale = float(np.sqrt(np.mean(np.sum(f_atm ** 2, axis=1))))
+                    for reg, mk in m.items():
+                        track["three_body_rms_error"][reg].append(
+                            float(np.sqrt(np.mean(np.sum((f_hyb[mk] - f_atm[mk]) ** 2, axis=1))) / scale))
+                if k < nseg:
+                    t0 = time.perf_counter()
+                    ctx.run(seg, kind3b=kind)
+                    timed += time.perf_counter() - t0
+            t = np.array(track["step"]) * 0.005
+            drift = {"total_slope_e_per_atom_per_tau": float(np.polyfit(t, track["energy_per_atom"], 1)[0]),
+                     "temperature_slope_per_tau": {reg: float(np.polyfit(t, track["temperature"][reg], 1)[0])
+                                                   for reg in track["temperature"]}}
             ctx.timing(True)
-            ctx.run(min(args.hybrid_steps, 20), kind3b=kind)
+            ctx.run(20, kind3b=kind)
             kern = {}
-            for k in ("cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
-                ms, cnt = ctx.timer(k)
+            for kk in ("cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
+                ms, cnt = ctx.timer(kk)
                 if cnt:
-                    kern[k] = round(ms / cnt * 1e3, 2)
+                    kern[kk] = round(ms / cnt * 1e3, 2)
             ctx.timing(False)
-            legs[name] = {"value": round(n * args.hybrid_steps / dt, 1),
-                          "ms_per_step": round(dt / args.hybrid_steps * 1e3, 4), "kernels_us": kern}
+            leg = {"value": round(n * seg * nseg / timed, 1), "ms_per_step": round(timed / (seg * nseg) * 1e3, 4),
+                   "kernels_us": kern, "drift": drift, "along_run": track}
+            if kind != K.HYBRID:
+                del leg["along_run"]["three_body_rms_error"]
+            legs[name] = leg
         out.update(legs)
+        m = region_masks(r0)
+        out["regions"] = {reg: {"sites_at_start": int(mk.sum()),
+                                "three_body_rms_error_mean": float(np.mean(legs["hybrid"]["along_run"]
+                                                                           ["three_body_rms_error"][reg])),
+                                "three_body_rms_error_max": float(np.max(legs["hybrid"]["along_run"]
+                                                                         ["three_body_rms_error"][reg])),
+                                "temperature_slope_per_tau": legs["hybrid"]["drift"]["temperature_slope_per_tau"][reg]}
+                          for reg, mk in m.items()}
         k = legs["hybrid"]["kernels_us"]
-        ns_exact = err["exact"]["sites"] + err["blend"]["sites"]
-        ns_sur = err["surrogate"]["sites"] + err["blend"]["sites"]
+        ns_exact = out["regions"]["exact"]["sites_at_start"] + out["regions"]["blend"]["sites_at_start"]
+        ns_sur = out["regions"]["surrogate"]["sites_at_start"] + out["regions"]["blend"]["sites_at_start"]
         # region throughput: the sites a region's three-body kernels serve over those kernels' time
         out["region_three_body_throughput"] = {
             "exact_atm_kernel": {"sites": ns_exact, "us": k.get("atm"),
@@ -298,6 +383,32 @@ def hybrid_leg(args):
         su = out["region_three_body_throughput"]["surrogate_kernels"]
         su["site_steps_per_s"] = round(ns_sur / (su["us"] * 1e-6), 1) if su["us"] else None
         out["speedup_hybrid_vs_exact"] = round(legs["hybrid"]["value"] / legs["exact_everywhere"]["value"], 3)
+    return out
+
+
+def drift_seeds(args):
+    """NVE energy drift of the exact and the surrogate path over several seeds of the
+    32,000-atom argon system (each equilibrated for equil_steps with the exact ATM, then
+    drift_steps of each path from the same state): mean and spread of the slopes."""
+    import komd_amd as K
+    from komd_amd import systems as S
+    res = {"exact": [], "surrogate": []}
+    for seed in range(1003, 1003 + args.drift_seeds):
+        sysm = S.fcc(20, 0.8, 0.8, seed)
+        with K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU) as ctx:
+            ctx.surrogate_set_params(np.load(WEIGHTS))
+            ctx.compute_forces(K.ATM_EXACT)
+            ctx.run(args.equil_steps, kind3b=K.ATM_EXACT)
+            r0, v0, _ = ctx.get_state()
+            for kind, name in ((K.ATM_EXACT, "exact"), (K.SURROGATE, "surrogate")):
+                ctx.set_state(r0, v0)
+                ctx.compute_forces(kind)
+                res[name].append(drift_of(ctx, K, sysm.n, kind, args.drift_steps)["slope_e_per_atom_per_tau"])
+    out = {"seeds": list(range(1003, 1003 + args.drift_seeds)), "steps": args.drift_steps}
+    for name, sl in res.items():
+        out[name] = {"slopes": sl, "mean": float(np.mean(sl)), "std": float(np.std(sl, ddof=1)) if len(sl) > 1 else 0.0}
+    out["difference_in_spreads"] = ((out["surrogate"]["mean"] - out["exact"]["mean"])
+                                    / max(out["exact"]["std"], 1e-300))
     return out
 
 
@@ -550,6 +661,7 @@ def main():
     if not args.no_hybrid and os.path.exists(WEIGHTS):
         hybrid = hybrid_leg(args)
     other = None if args.no_other else other_legs(args)
+    seeds = drift_seeds(args) if args.drift_seeds > 0 and args.drift_steps > 0 and os.path.exists(WEIGHTS) else None
 
     cpu, force_error = cpu_reference_leg(sysm, snapshot, args)
     drift_fp64 = None if args.drift_steps <= 0 else drift_against_fp64(K)
@@ -573,6 +685,7 @@ def main():
         "kernels": kernels,
         "learned": learned or None,
         "hybrid_slab": hybrid,
+        "energy_drift_seeds": seeds,
         "two_body_configs": other,
     }
     print(json.dumps(out), flush=True)
