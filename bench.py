@@ -23,9 +23,9 @@ Prints one JSON line:
                snapshot, total force and three-body force separately (the FP64
                forces come from the same CPU leg as cpu_baseline)
   energy_drift NVE drift of the total energy per atom over --drift-steps steps
-  energy_drift_vs_fp64
+  energy_drift_vs_fp64, energy_drift_vs_fp64_32000
                the GPU path's drift beside the FP64 run of the same 4,000-atom
-               LJ + ATM system over 10,000 steps (stored fixture)
+               and 32,000-atom LJ + ATM systems over 10,000 steps (stored fixtures)
   learned      the learned three-body path (descriptor-MLP surrogate, weights in
                komd_amd/weights/) at 32,000 and 256,000 atoms: throughput beside the
                exact path on the same state, its three-body force error against the
@@ -59,7 +59,7 @@ HBM_PEAK = 8.0e12  # B/s
 #   kick        (thermostatted steps only) Integrator::Step1: read v f mass 56, write v 24
 KERNEL_BYTES_PER_SITE = {"bin_key": 160, "bin_scatter": 12, "bin_sort": 56, "kick": 80}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "atm_32000_traffic.json")
-DRIFT_FIXTURE = os.path.join(ROOT, "tests", "golden", "drift_fp64_fcc4000_atm.json")
+DRIFT_FIXTURES = {n: os.path.join(ROOT, "tests", "golden", f"drift_fp64_fcc{n}_atm.json") for n in (4000, 32000)}
 JUDGE_EVERY, JUDGE_MAX = 100, 0.06  # exact-ATM check of the surrogate: period (steps) and RMS bound
 WEIGHTS = os.path.join(ROOT, "komd_amd", "weights", "atm_surrogate.npy")
 RHO = 0.8
@@ -204,7 +204,16 @@ def learned_leg(n_atoms, args):
             t0 = time.perf_counter()
             ctx.run(args.steps, kind3b=0)
             dt = time.perf_counter() - t0
-            comp[name] = {"value": round(n * args.steps / dt, 1), "ms_per_step": round(dt / args.steps * 1e3, 4)}
+            ctx.timing(True)
+            ctx.run(min(args.steps, 200), kind3b=0)
+            kern = {}
+            for k in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj", "cell_nl", "table", "limit"):
+                ms, cnt = ctx.timer(k)
+                if cnt:
+                    kern[k] = round(ms / cnt * 1e3, 2)
+            ctx.timing(False)
+            comp[name] = {"value": round(n * args.steps / dt, 1), "ms_per_step": round(dt / args.steps * 1e3, 4),
+                          "kernels_us": kern}
         ctx.select_pair(K.PAIR_LJ12_6)
         out["comparator_throughput"] = comp
         legs = {}
@@ -506,17 +515,18 @@ def cpu_reference_leg(sysm, snapshot, args):
     return cpu, force_error
 
 
-def drift_against_fp64(K):
+def drift_against_fp64(K, n_atoms):
     """NVE energy trajectory of the GPU path against the FP64 run of the same system
-    (4,000 atoms FCC 10^3 x 4, rho 0.8, T* 0.8, LJ + complete ATM, 10,000 steps), whose
-    E/N every 100 steps is stored in tests/golden/drift_fp64_fcc4000_atm.json (written by
-    tests/golden/make_drift_fp64.py on the CPU)."""
+    (n_atoms FCC argon, rho 0.8, T* 0.8, seed 1003, LJ + complete ATM, 10,000 steps), whose
+    E/N every 100 steps is stored in tests/golden/drift_fp64_fcc<n>_atm.json (written by
+    tests/golden/make_drift_fp64.py on the CPU: 4,000 atoms and the 32,000-atom workload)."""
     from komd_amd import systems as S
-    if not os.path.exists(DRIFT_FIXTURE):
+    path = DRIFT_FIXTURES[n_atoms]
+    if not os.path.exists(path):
         return None
-    with open(DRIFT_FIXTURE) as fh:
+    with open(path) as fh:
         ref = json.load(fh)
-    s = S.fcc(10, 0.8, 0.8, 1003)
+    s = S.fcc(int(round((n_atoms / 4) ** (1 / 3))), 0.8, 0.8, 1003)
     every, steps = ref["every"], ref["steps"]
     e = []
     with K.Context(s, cutoff=2.5, dt=0.005, nu=NU) as ctx:
@@ -664,7 +674,8 @@ def main():
     seeds = drift_seeds(args) if args.drift_seeds > 0 and args.drift_steps > 0 and os.path.exists(WEIGHTS) else None
 
     cpu, force_error = cpu_reference_leg(sysm, snapshot, args)
-    drift_fp64 = None if args.drift_steps <= 0 else drift_against_fp64(K)
+    drift_fp64 = None if args.drift_steps <= 0 else drift_against_fp64(K, 4000)
+    drift_fp64_32k = None if args.drift_steps <= 0 else drift_against_fp64(K, 32000)
 
     cells = "20^3" if n == 32000 else "40^3"
     out = {
@@ -681,6 +692,7 @@ def main():
         "cpu_baseline": cpu,
         "force_error": force_error,
         "energy_drift_vs_fp64": drift_fp64,
+        "energy_drift_vs_fp64_32000": drift_fp64_32k,
         "energy_drift": drift,
         "kernels": kernels,
         "learned": learned or None,

@@ -334,13 +334,15 @@ def test_limit_clamps_components():
     assert np.all(g[:, 0] == 80.0) and np.all(g[:, 1] == -80.0) and np.all(g[:, 2] == 3.0)
 
 
-def test_drift_fixture_consistent():
-    """The stored FP64 energy trajectory (tests/golden/make_drift_fp64.py) starts at the
-    oracle's energy of the generated state, and its slope is the fit of its own samples."""
+@pytest.mark.parametrize("cells", [10, 20])
+def test_drift_fixture_consistent(cells):
+    """The stored FP64 energy trajectories (tests/golden/make_drift_fp64.py; 4,000 atoms and
+    the 32,000-atom workload) start at the oracle's energy of the generated state, and their
+    slopes are the fits of their own samples."""
     import json
-    with open(os.path.join(FIXTURES, "drift_fp64_fcc4000_atm.json")) as fh:
+    s = S.fcc(cells, 0.8, 0.8, 1003)
+    with open(os.path.join(FIXTURES, f"drift_fp64_fcc{s.n}_atm.json")) as fh:
         ref = json.load(fh)
-    s = S.fcc(10, 0.8, 0.8, 1003)
     r, b = O.update_container(s)
     _, u, _ = O.lj_c08(s, r, b, energy=True, nthreads=4)
     _, ea, _ = O.atm_complete(s, r, b, NU, coverage=0, sign=-1.0, nthreads=4)
