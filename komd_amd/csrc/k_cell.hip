@@ -83,6 +83,56 @@ struct CellArgs {
     double rc2, limit, kick;
 };
 
+// Wave 0 of a workgroup (lanes 0..26 carry a neighbour cell each): the 27 cells around
+// `cell` in (dz, dy, dx) order, dx fastest -- start of their sites, the prefix of their
+// counts (nb_pref[27] = sites of the neighbourhood), the periodic image offset of each
+// relative to the home cell centre (FP32) and its image code in the site word.
+__device__ __forceinline__ void nb_setup(const Grid& g, const uint32_t* __restrict__ start, int cell, int tid,
+                                         int* nb_start, int* nb_pref, float (*nb_off)[3], uint32_t* nb_code) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int hx = cell % nx, hy = (cell / nx) % ny, hz = cell / (nx * ny);
+    int cnt = 0;
+    if (tid < 27) {
+        const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
+        int x = hx + dx, y = hy + dy, z = hz + dz;
+        uint32_t cx = 0, cy = 0, cz = 0;
+        double sx = 0, sy = 0, sz = 0;
+        if (x < 0) { x += nx; cx = 2; sx = -g.size[0]; } else if (x >= nx) { x -= nx; cx = 1; sx = g.size[0]; }
+        if (y < 0) { y += ny; cy = 2; sy = -g.size[1]; } else if (y >= ny) { y -= ny; cy = 1; sy = g.size[1]; }
+        if (z < 0) { z += nz; cz = 2; sz = -g.size[2]; } else if (z >= nz) { z -= nz; cz = 1; sz = g.size[2]; }
+        const int nc = x + nx * (y + ny * z);
+        nb_start[tid] = (int)start[nc];
+        cnt = (int)(start[nc + 1] - start[nc]);
+        nb_off[tid][0] = (float)((double)(x - hx) * g.cs + sx);
+        nb_off[tid][1] = (float)((double)(y - hy) * g.cs + sy);
+        nb_off[tid][2] = (float)((double)(z - hz) * g.cs + sz);
+        nb_code[tid] = (cx | (cy << 2) | (cz << 4)) << kIdxBits;
+    }
+    int incl = cnt;  // inclusive scan across the wave
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (tid >= off) incl += o;
+    }
+    if (tid < 27) nb_pref[tid + 1] = incl;
+    if (tid == 0) nb_pref[0] = 0;
+}
+
+// the neighbour cell holding the gidx-th site of the neighbourhood (binary search of the prefix)
+__device__ __forceinline__ int nb_of(const int* nb_pref, int gidx) {
+    int k = 0;
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1)
+        if (k + step < 27 && nb_pref[k + step] <= gidx) k += step;
+    return k;
+}
+
+// a neighbour's packed position relative to the home cell centre: image offset folded
+// into the FP32 coordinates, image code into the site word
+__device__ __forceinline__ float4 staged(float4 v, const float* off, uint32_t code) {
+    return make_float4(v.x + off[0], v.y + off[1], v.z + off[2], __uint_as_float(__float_as_uint(v.w) | code));
+}
+
 // NL: 0 none, 1 neighbour list, 2 neighbour list + cell directions.
 template <bool UNIFORM, bool MOLS, bool LJ, int NL, bool ENERGY, bool LIMIT>
 __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
@@ -130,35 +180,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     const bool live = cell < a.g.ncell;
     int m = 0, hs = 0, nh = 0;
     if (live) {
-        const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-        const int hx = cell % nx, hy = (cell / nx) % ny, hz = cell / (nx * ny);
-        if (tid < 64) {  // wave 0: the 27 neighbour cells and the prefix of their counts
-            int cnt = 0;
-            if (tid < 27) {
-                const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
-                int x = hx + dx, y = hy + dy, z = hz + dz;
-                uint32_t cx = 0, cy = 0, cz = 0;
-                double sx = 0, sy = 0, sz = 0;
-                if (x < 0) { x += nx; cx = 2; sx = -a.g.size[0]; } else if (x >= nx) { x -= nx; cx = 1; sx = a.g.size[0]; }
-                if (y < 0) { y += ny; cy = 2; sy = -a.g.size[1]; } else if (y >= ny) { y -= ny; cy = 1; sy = a.g.size[1]; }
-                if (z < 0) { z += nz; cz = 2; sz = -a.g.size[2]; } else if (z >= nz) { z -= nz; cz = 1; sz = a.g.size[2]; }
-                const int nc = x + nx * (y + ny * z);
-                nb_start[tid] = (int)a.start[nc];
-                cnt = (int)(a.start[nc + 1] - a.start[nc]);
-                nb_off[tid][0] = (float)((double)(x - hx) * a.g.cs + sx);
-                nb_off[tid][1] = (float)((double)(y - hy) * a.g.cs + sy);
-                nb_off[tid][2] = (float)((double)(z - hz) * a.g.cs + sz);
-                nb_code[tid] = (cx | (cy << 2) | (cz << 4)) << kIdxBits;
-            }
-            int incl = cnt;  // inclusive scan across the wave
-#pragma unroll
-            for (int off = 1; off < 32; off <<= 1) {
-                const int o = __shfl_up(incl, off);
-                if (tid >= off) incl += o;
-            }
-            if (tid < 27) nb_pref[tid + 1] = incl;
-            if (tid == 0) nb_pref[0] = 0;
-        }
+        if (tid < 64) nb_setup(a.g, a.start, cell, tid, nb_start, nb_pref, nb_off, nb_code);
         __syncthreads();
         // workgroup-uniform values read from LDS: keep them in scalar registers
         m = __builtin_amdgcn_readfirstlane(nb_pref[27]);
@@ -194,17 +216,14 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
         uint32_t ncnt = 0;
         for (int c0 = 0; c0 < m; c0 += kStage) {
             const int mc = min(kStage, m - c0);
-            __syncthreads();
+            // the first staging needs no barrier (the stage is untouched since the
+            // neighbour-cell barrier), so the home-site loads above overlap its loads
+            if (base > 0 || c0 > 0) __syncthreads();
             for (int t = tid; t < mc; t += kCellThreads) {
                 const int gidx = c0 + t;
-                int k = 0;
-#pragma unroll
-                for (int step = 16; step > 0; step >>= 1)
-                    if (k + step < 27 && nb_pref[k + step] <= gidx) k += step;
+                const int k = nb_of(nb_pref, gidx);
                 const int j = nb_start[k] + (gidx - nb_pref[k]);
-                const float4 v = a.spos[j];
-                pos[t] = make_float4(v.x + nb_off[k][0], v.y + nb_off[k][1], v.z + nb_off[k][2],
-                                     __uint_as_float(__float_as_uint(v.w) | nb_code[k]));
+                pos[t] = staged(a.spos[j], nb_off[k], nb_code[k]);
                 if (!UNIFORM) cprm[t] = a.sprm[j];
                 if (MOLS) cmol[t] = a.sid[j];
                 if (NL == 2) sdir[t] = (uint16_t)(k | ((gidx - nb_pref[k]) << 5));
@@ -297,13 +316,14 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
 
     // in-range pair count (the same in every lane of a wave): one slot per workgroup,
     // accumulated across launches
-    // (a single global counter would serialise thousands of same-address atomics)
+    // (a single global counter would serialise thousands of same-address atomics; an
+    // atomic add without return leaves the workgroup at once, where += waits for a load)
     if (lane == 0) red_cnt[wv] = npairs;
     __syncthreads();
     if (tid == 0) {
         unsigned long long t = 0;
         for (int k = 0; k < kWaves; k++) t += red_cnt[k];
-        a.counters[b] += t;
+        atomicAdd(&a.counters[b], t);
     }
     if (ENERGY) {
         u_acc = wave_sum(u_acc);
@@ -583,7 +603,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
         for (int k = 0; k < kAtmWaves; k++) t += red_cnt[k];
-        a.counters[b] += t;
+        atomicAdd(&a.counters[b], t);  // no return: the workgroup does not wait on it
         if (ENERGY) {
             double s = 0;
             for (int k = 0; k < kAtmWaves; k++) s += red[k];
@@ -627,6 +647,8 @@ hipError_t dispatch(komd_ctx* c, const CellArgs& a, bool lj, int nl, bool energy
 // `cell_split` workgroups per cell on small grids.  Otherwise XCD x owns cx consecutive cells;
 // with S < n < 2S cells for S resident slots, 2S - n of them get one workgroup and the rest
 // two (a partial second round of half-cells instead of whole ones); otherwise one each.
+// (Measured at 32k: the last n - S cells in S / (n - S) short pieces each, after S whole
+// cells, took 24.9 us against 23.1: the pieces repeat the neighbourhood staging.)
 static void cell_jobs(const komd_ctx* c, int* cx, int* chunk) {
     const int C = c->grid.ncell;
     if (c->cell_split > 1 || c->xcd_slots <= 0) {

@@ -198,18 +198,26 @@ def test_step_forces_lj_in_three_body_kernel(orc, cells, seed, rho):
     f_atm, _, _ = orc.atm_complete(s, r, b, NU, coverage=0, sign=-1.0, nthreads=8)
     with K.Context(s, nu=NU) as c, K.Context(s, nu=NU) as d:
         c.update_container()
+        c.counters(reset=True)
         c.compute_forces(K.ATM_EXACT)
         f = c.forces()
+        cnt = c.counters()
         d.update_container()
+        d.counters(reset=True)
         d.forces_pair()
+        d.counters(reset=True)  # the pair pass counts too: keep the three-body pass's counts only
         d.forces_3b(K.ATM_EXACT)
         f_sep = d.forces()
+        cnt_sep = d.counters()
     mx, rms = rel_err(f, f_lj + f_atm)
     mx2, rms2 = rel_err(f, f_sep)
     print(f"step forces vs oracle max {mx:.3e} rms {rms:.3e}; vs separate passes max {mx2:.3e} rms {rms2:.3e}")
     assert mx < 2e-4 and rms < 5e-6
     # the same FP32 pair terms, summed in FP64 in another order (r^2 may differ by an ulp)
     assert mx2 < 1e-5 and rms2 < 1e-6
+    # the step builds each site's list inside the three-body kernel: the same pair set
+    # (unique in-range pairs) and the same triplet visits as the list pass + ATM pass
+    assert cnt == cnt_sep, (cnt, cnt_sep)
 
 
 def test_atm_exact_long_lists(orc):
@@ -242,6 +250,12 @@ def test_neighbour_list_overflow_is_reported(orc):
         c.update_container()
         with pytest.raises(K.KomdError, match="neighbour-list capacity"):
             c.forces_3b(K.ATM_EXACT)
+            c.forces()
+        c.forces_pair()
+        c.forces()
+        # the NVE step's force pass builds the lists inside the three-body kernel
+        with pytest.raises(K.KomdError, match="neighbour-list capacity"):
+            c.compute_forces(K.ATM_EXACT)
             c.forces()
         c.forces_pair()
         c.forces()
