@@ -565,7 +565,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the force path is single-GPU here (spatial decomposition "
-              "over several GPUs is described in DESIGN.md but not built); run with --gpus 1", file=sys.stderr)
+              "over several GPUs is designed but not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_mib:
         copy_calibration(args.copy_mib)

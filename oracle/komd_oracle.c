@@ -439,8 +439,8 @@ EXPORT double or_atm_comp_force(double nu, double r_ab, double r_ac, double r_bc
 }
 
 /* Textbook triple-dipole energy V = nu (1 + 3 cos g1 cos g2 cos g3) / (r12 r13 r23)^3,
- * written with the law of cosines (no reference counterpart: the reference has no
- * ATM energy). */
+ * written with the law of cosines (no reference counterpart: the reference computes no
+ * ATM energy; ATM.cpp and ATM_NOLIST.cpp only add forces). */
 EXPORT double or_atm_energy_triplet(double nu, const double* ri, const double* rj, const double* rk) {
     double a2 = 0, b2 = 0, c2 = 0;
     for (int d = 0; d < 3; d++) {

@@ -235,7 +235,7 @@ def test_atm_reference_walk_equals_independent_coverage_rule():
         f, _ = O.atm_c08(s, r, b, NU, sign=1.0)
         fb, _, hb = O.atm_brute(s, r, NU, site_cell=b.site_cell, coverage=1, sign=1.0)
         assert np.abs(f - fb).max() < 1e-12 * max(1.0, np.abs(fb).max())
-        # and the skipped 2+1 share is large at this density (SURVEY measured 48.5 %)
+        # and the skipped 2+1 share is large at this density (about half of all triplets)
         _, _, hall = O.atm_brute(s, r, NU, coverage=0, sign=1.0)
         assert 0.3 < 1 - hb / hall < 0.7
 
