@@ -229,6 +229,22 @@ komd_status komd_run(komd_ctx* ctx, int64_t nsteps, int32_t kind3b, int32_t use_
  * (the pre-loop pass of Simulation.cpp:15-24). */
 komd_status komd_compute_forces(komd_ctx* ctx, int32_t kind3b, int32_t use_bonded);
 
+/* Verlet lists for komd_run, the reference's reuse of its pair list over several steps
+ * (LinkedCells::updatePairList every pair_steps steps, PairList.h:22-24, LinkedCells.cpp:130-131)
+ * made exact: the list holds every site within cutoff + skin at its last rebuild, the force
+ * kernel tests the cutoff at the current positions each step (the same FP32 test and FP64
+ * decision near the cutoff as a fresh list), and the device rebuilds the container and the
+ * lists before any site has moved more than skin/2 since the last rebuild.  Every step's
+ * interaction set therefore equals the one of a container update at that step.  Applies to
+ * komd_run with KOMD_3B_ATM_EXACT, LJ 12-6 of uniform single-site sites without the clamp,
+ * no bonded terms and no thermostat (t0 <= 0); other runs ignore it.  The run's last
+ * container update wraps the positions and leaves the grid consistent with them.
+ * skin = 0 (default) turns it off; skin > 0 needs cell_size >= cutoff + skin. */
+komd_status komd_set_skin(komd_ctx* ctx, double skin);
+/* Container updates with list rebuilds of skinned runs since the last komd_set_skin (the
+ * first step of every skinned komd_run is one). */
+komd_status komd_skin_rebuilds(komd_ctx* ctx, int64_t* rebuilds);
+
 /* Per-kernel device time measured with HIP events on the context's stream.
  * enable = 1 starts accumulating (komd_run then launches without a graph). */
 komd_status komd_timing_enable(komd_ctx* ctx, int32_t enable);

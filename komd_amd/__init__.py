@@ -107,6 +107,8 @@ _SIGNATURES = {
     "komd_set_judge": (C.c_int, [_P, C.c_int64, C.c_double]),
     "komd_judge_report": (C.c_int, [_P, _P, _P, _P, _P]),
     "komd_set_hybrid": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_double]),
+    "komd_set_skin": (C.c_int, [_P, C.c_double]),
+    "komd_skin_rebuilds": (C.c_int, [_P, _P]),
 }
 
 _LIB = None
@@ -330,6 +332,17 @@ class Context:
 
     def run(self, nsteps: int, kind3b: int = 0, bonded: bool = False, t0: float = 0.0):
         self._check(self.lib.komd_run(self._ctx, nsteps, kind3b, 1 if bonded else 0, t0))
+
+    def set_skin(self, skin: float):
+        """Verlet lists of width cutoff + skin for komd_run (exact ATM + LJ, NVE); 0: off.
+        Needs cell_size >= cutoff + skin."""
+        self._check(self.lib.komd_set_skin(self._ctx, float(skin)))
+
+    def skin_rebuilds(self) -> int:
+        """List rebuilds of skinned runs since set_skin."""
+        out = C.c_int64(0)
+        self._check(self.lib.komd_skin_rebuilds(self._ctx, C.byref(out)))
+        return int(out.value)
 
     def observables(self, kind3b: int = 0) -> dict:
         o = Observables()

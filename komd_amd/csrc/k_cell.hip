@@ -64,6 +64,9 @@ struct CellArgs {
     float4* __restrict__ nl;       // [N][kFullCap] neighbour list of every site (cell-sorted order)
     uint32_t* __restrict__ nlc;    // [N] its length
     uint16_t* __restrict__ nld;    // [N][kFullCap] neighbour cell direction | rank in that cell << 5
+    uint32_t* __restrict__ vl;     // NL 3: [N][kSkinCap] Verlet list (cell-sorted slot | direction << 26)
+    uint32_t* __restrict__ vlc;    // NL 3: [N] its length
+    const uint32_t* __restrict__ req;  // NL 3: rebuild word of the step (0: the lists stay)
     double* __restrict__ part;
     double* __restrict__ vpart;    // [kSym][nblocks] virial tensor partials (energy mode)
     uint32_t* __restrict__ flags;
@@ -79,34 +82,58 @@ struct CellArgs {
     // workgroups each (placed last), so the second round holds half-cells only
     int cx, slots;
     float rc2f, band, sig2, eps24, eps4;
+    float rl2f;     // NL 3: (cutoff + skin)^2
     float c24, c4;  // 24 and 4 times the pair-force scale (mixed sites: e24 = c24 eps_ij)
     double rc2, limit, kick;
 };
 
-// Wave 0 of a workgroup (lanes 0..26 carry a neighbour cell each): the 27 cells around
-// `cell` in (dz, dy, dx) order, dx fastest -- start of their sites, the prefix of their
-// counts (nb_pref[27] = sites of the neighbourhood), the periodic image offset of each
-// relative to the home cell centre (FP32) and its image code in the site word.
-__device__ __forceinline__ void nb_setup(const Grid& g, const uint32_t* __restrict__ start, int cell, int tid,
-                                         int* nb_start, int* nb_pref, float (*nb_off)[3], uint32_t* nb_code) {
+// Per axis, the offset of a neighbour cell falls in one of five classes: 0 same cell
+// column, 1 / 2 the cell below / above, 3 / 4 the periodic image below / above (the home
+// cell is the first / last of the axis).  A Verlet-list entry carries the triple as one
+// number cx + 5 (cy + 5 cz) < 125.
+__device__ __forceinline__ float axis_offset(int cls, int n, double cs, double L) {
+    const int d = cls == 0 ? 0 : (cls == 1 ? -1 : (cls == 2 ? 1 : (cls == 3 ? n - 1 : -(n - 1))));
+    const double s = cls == 3 ? -L : (cls == 4 ? L : 0.0);
+    return (float)((double)d * cs + s);  // the FP64 expression of nb_cell
+}
+
+// Neighbour cell `dir` (0..26, (dz, dy, dx) order, dx fastest) of `cell`: its flat index,
+// the periodic image offset relative to the home cell centre (FP32 of the FP64 value),
+// its image code in the site word and (cls) its offset class.
+__device__ __forceinline__ int nb_cell(const Grid& g, int cell, int dir, float* off, uint32_t* code,
+                                       uint32_t* cls = nullptr) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int hx = cell % nx, hy = (cell / nx) % ny, hz = cell / (nx * ny);
+    const int dx = dir % 3 - 1, dy = (dir / 3) % 3 - 1, dz = dir / 9 - 1;
+    int x = hx + dx, y = hy + dy, z = hz + dz;
+    uint32_t cx = 0, cy = 0, cz = 0;
+    double sx = 0, sy = 0, sz = 0;
+    if (x < 0) { x += nx; cx = 2; sx = -g.size[0]; } else if (x >= nx) { x -= nx; cx = 1; sx = g.size[0]; }
+    if (y < 0) { y += ny; cy = 2; sy = -g.size[1]; } else if (y >= ny) { y -= ny; cy = 1; sy = g.size[1]; }
+    if (z < 0) { z += nz; cz = 2; sz = -g.size[2]; } else if (z >= nz) { z -= nz; cz = 1; sz = g.size[2]; }
+    off[0] = (float)((double)(x - hx) * g.cs + sx);
+    off[1] = (float)((double)(y - hy) * g.cs + sy);
+    off[2] = (float)((double)(z - hz) * g.cs + sz);
+    *code = (cx | (cy << 2) | (cz << 4)) << kIdxBits;
+    if (cls) {
+        auto c1 = [](int d, uint32_t img) { return img == 2u ? 3u : (img == 1u ? 4u : (d < 0 ? 1u : (d > 0 ? 2u : 0u))); };
+        *cls = c1(dx, cx) + 5u * (c1(dy, cy) + 5u * c1(dz, cz));
+    }
+    return x + nx * (y + ny * z);
+}
+
+// Wave 0 of a workgroup (lanes 0..26 carry a neighbour cell each): the 27 cells around
+// `cell` -- start of their sites, the prefix of their counts (nb_pref[27] = sites of the
+// neighbourhood), the periodic image offset of each relative to the home cell centre
+// (FP32) and its image code in the site word.
+__device__ __forceinline__ void nb_setup(const Grid& g, const uint32_t* __restrict__ start, int cell, int tid,
+                                         int* nb_start, int* nb_pref, float (*nb_off)[3], uint32_t* nb_code,
+                                         uint32_t* nb_cls) {
     int cnt = 0;
     if (tid < 27) {
-        const int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
-        int x = hx + dx, y = hy + dy, z = hz + dz;
-        uint32_t cx = 0, cy = 0, cz = 0;
-        double sx = 0, sy = 0, sz = 0;
-        if (x < 0) { x += nx; cx = 2; sx = -g.size[0]; } else if (x >= nx) { x -= nx; cx = 1; sx = g.size[0]; }
-        if (y < 0) { y += ny; cy = 2; sy = -g.size[1]; } else if (y >= ny) { y -= ny; cy = 1; sy = g.size[1]; }
-        if (z < 0) { z += nz; cz = 2; sz = -g.size[2]; } else if (z >= nz) { z -= nz; cz = 1; sz = g.size[2]; }
-        const int nc = x + nx * (y + ny * z);
+        const int nc = nb_cell(g, cell, tid, nb_off[tid], &nb_code[tid], &nb_cls[tid]);
         nb_start[tid] = (int)start[nc];
         cnt = (int)(start[nc + 1] - start[nc]);
-        nb_off[tid][0] = (float)((double)(x - hx) * g.cs + sx);
-        nb_off[tid][1] = (float)((double)(y - hy) * g.cs + sy);
-        nb_off[tid][2] = (float)((double)(z - hz) * g.cs + sz);
-        nb_code[tid] = (cx | (cy << 2) | (cz << 4)) << kIdxBits;
     }
     int incl = cnt;  // inclusive scan across the wave
 #pragma unroll
@@ -133,19 +160,23 @@ __device__ __forceinline__ float4 staged(float4 v, const float* off, uint32_t co
     return make_float4(v.x + off[0], v.y + off[1], v.z + off[2], __uint_as_float(__float_as_uint(v.w) | code));
 }
 
-// NL: 0 none, 1 neighbour list, 2 neighbour list + cell directions.
+// NL: 0 none, 1 neighbour list, 2 neighbour list + cell directions, 3 Verlet list
+// (every site within cutoff + skin; the step's rebuild word gates the launch).
 template <bool UNIFORM, bool MOLS, bool LJ, int NL, bool ENERGY, bool LIMIT>
 __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
+    if (NL == 3 && a.req[0] == 0u) return;
     constexpr int kStage = kChunkLJ;
     constexpr int kWaves = kCellThreads / 64;
     __shared__ float4 pos[kStage + 32];  // + 32: a group's lanes past the chunk read in bounds
     __shared__ uint16_t sdir[NL == 2 ? kStage : 1];
+    __shared__ uint32_t sref[NL == 3 ? kStage : 1];  // cell-sorted slot | offset class << 25
     __shared__ float2 cprm[UNIFORM ? 1 : kStage];
     __shared__ uint32_t cmol[MOLS ? kStage : 1];
     __shared__ int nb_start[27];
     __shared__ int nb_pref[28];
     __shared__ float nb_off[27][3];
     __shared__ uint32_t nb_code[27];
+    __shared__ uint32_t nb_cls[27];
     __shared__ double red[2][kWaves];
     __shared__ unsigned long long red_cnt[kWaves];
 
@@ -180,7 +211,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
     const bool live = cell < a.g.ncell;
     int m = 0, hs = 0, nh = 0;
     if (live) {
-        if (tid < 64) nb_setup(a.g, a.start, cell, tid, nb_start, nb_pref, nb_off, nb_code);
+        if (tid < 64) nb_setup(a.g, a.start, cell, tid, nb_start, nb_pref, nb_off, nb_code, nb_cls);
         __syncthreads();
         // workgroup-uniform values read from LDS: keep them in scalar registers
         m = __builtin_amdgcn_readfirstlane(nb_pref[27]);
@@ -227,6 +258,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 if (!UNIFORM) cprm[t] = a.sprm[j];
                 if (MOLS) cmol[t] = a.sid[j];
                 if (NL == 2) sdir[t] = (uint16_t)(k | ((gidx - nb_pref[k]) << 5));
+                if (NL == 3) sref[t] = (uint32_t)j | (nb_cls[k] << kSkinIdxBits);
             }
             __syncthreads();
             // every lane of the workgroup runs the same trip count (ballots need the whole wave)
@@ -240,14 +272,28 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 // lanes without a candidate test +inf: every predicate below is then one
                 // compare, which the ballots take as a lane mask directly
                 const float r2 = cand ? dx * dx + dy * dy + dz * dz : __builtin_huge_valf();
-                bool in = r2 <= a.rc2f;
-                // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
-                const bool amb = fabsf(r2 - a.rc2f) <= a.band;
-                if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
+                bool in;
+                if (NL == 3) {
+                    // Verlet list: a superset bound, no exact decision needed
+                    in = r2 <= a.rl2f;
+                } else {
+                    in = r2 <= a.rc2f;
+                    // a distance within a few ulps of rc is decided in FP64 (rare: wave-uniform branch)
+                    const bool amb = fabsf(r2 - a.rc2f) <= a.band;
+                    if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
+                }
                 const uint32_t oh = in ? oj : oi;  // hit: in range and not i itself
                 const bool hit = oh != oi;
-                npairs += (unsigned long long)__popcll(ballot(oh > oi));  // wave-uniform
-                if (NL) {
+                if (NL != 3) npairs += (unsigned long long)__popcll(ballot(oh > oi));  // wave-uniform
+                if (NL == 3) {
+                    const unsigned long long bal = ballot(hit) & gmask;
+                    if (hit) {
+                        const uint32_t s = ncnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                        if (s < (uint32_t)kSkinCap) a.vl[(size_t)p * kSkinCap + s] = sref[t];
+                    }
+                    ncnt += (uint32_t)__popcll(bal);
+                } else if (NL) {
                     const unsigned long long bal = ballot(hit) & gmask;
                     if (hit) {
                         const uint32_t s = ncnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
@@ -288,7 +334,10 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 }
             }
         }
-        if (NL && active && sub == 0) {
+        if (NL == 3 && active && sub == 0) {
+            a.vlc[p] = min(ncnt, (uint32_t)kSkinCap);
+            if (ncnt > (uint32_t)kSkinCap) atomicOr(a.flags, FLAG_VL_OVERFLOW);
+        } else if (NL && active && sub == 0) {
             a.nlc[p] = min(ncnt, (uint32_t)kFullCap);
             if (ncnt > (uint32_t)kFullCap) atomicOr(a.flags, FLAG_NL_OVERFLOW);
         }
@@ -365,50 +414,140 @@ struct AtmArgs {
     double* __restrict__ vy;
     double* __restrict__ vz;
     const double* __restrict__ mass;
+    // Verlet-list steps (SKIN): the list, the FP32 cell-relative positions now and at the
+    // last rebuild, and the next step's rebuild word
+    const uint32_t* __restrict__ vl;
+    const uint32_t* __restrict__ vlc;
+    const float4* __restrict__ spos;
+    const float4* __restrict__ spb;
+    uint32_t* __restrict__ req_next;
+    unsigned long long* __restrict__ pair_counters;  // in-range unique pairs per workgroup slot
+    uint32_t* __restrict__ flags;
+    double dt, lim2;                   // time step; squared displacement that triggers a rebuild
     Grid g;
     int n, chunk, nblocks, hyb_axis;
     int h16;                           // 1e-2 <= rc <= 1e4: the FP16 rotation filter is exact enough
     int lj;                            // also the LJ force of the site's list (uniform sites, no clamp)
     float sig2, eps24;
-    float rc2f, band3, nu;
+    float rc2f, band, band3, nu;
     double rc2, kick;
 };
+
+// Would the next Integrator::Step0 (Integrator.cpp:28-32) move site i more than sqrt(lim2)
+// from its build position?  The step (v + f dt/(2m)) dt of the drift, after this step's
+// second half-kick has been written to v, plus the displacement since the rebuild from the
+// FP32 cell-relative positions (both relative to the same build-time cell centre; their
+// rounding, below 3e-7, lies inside the margin of lim2).
+__device__ __forceinline__ bool skin_exceeded(const double* __restrict__ vx, const double* __restrict__ vy,
+                                              const double* __restrict__ vz, const double* __restrict__ mass,
+                                              uint32_t i, double dt_half, double dt, double fx, double fy, double fz,
+                                              float4 q, float4 qb, double lim2) {
+    const double k = dt_half / mass[i];
+    const double x = (double)(q.x - qb.x) + (vx[i] + fx * k) * dt;
+    const double y = (double)(q.y - qb.y) + (vy[i] + fy * k) * dt;
+    const double z = (double)(q.z - qb.z) + (vz[i] + fz * k) * dt;
+    return !(x * x + y * y + z * z <= lim2);
+}
 
 constexpr int kAtmWaves = 4;
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-// ATM: 1 physical, 2 reference semantics.
-template <int ATM, bool ENERGY, bool HYB>
+// ATM: 1 physical, 2 reference semantics.  SKIN (physical, force pass, no hybrid): the
+// site's in-range neighbours come from its Verlet list and the current positions; the
+// kernel then writes the complete force (it is the step's only force kernel) and raises
+// the next step's rebuild word when the next drift would leave the skin.
+template <int ATM, bool ENERGY, bool HYB, bool SKIN = false>
 __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_atm(AtmArgs a) {
+    static_assert(!SKIN || (ATM == 1 && !ENERGY && !HYB), "Verlet lists: physical ATM force pass only");
     __shared__ float4 wbuf[kAtmWaves][kFullCap];
+    __shared__ float4 woff[SKIN ? 125 : 1];  // offset of each Verlet-list offset class, image code in .w
     __shared__ float2 wsq[kAtmWaves][kFullCap];  // |x_a|^2 and its reciprocal per neighbour
     __shared__ uint32_t qbuf[kAtmWaves][128];  // ring of 128 candidate pairs
     __shared__ uint8_t wdir[ATM == 2 ? kAtmWaves : 1][ATM == 2 ? kFullCap : 1];
     __shared__ float wlam[HYB ? kAtmWaves : 1][HYB ? kFullCap : 1];
     __shared__ double red[kAtmWaves];
     __shared__ unsigned long long red_cnt[kAtmWaves];
+    __shared__ unsigned long long red_np[SKIN ? kAtmWaves : 1];
 
     const int lane = threadIdx.x & 63;
     const int wvu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = blockIdx.x;
     const int lb = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous site slabs per XCD
     const int p = lb * kAtmWaves + wvu;
+    if (SKIN) {
+        // the 125 offset classes of the Verlet-list entries (the list pass's FP32 offsets)
+        const int t = threadIdx.x;
+        if (t < 125) {
+            const int cx = t % 5, cy = (t / 5) % 5, cz = t / 25;
+            auto img = [](int c) { return c == 3 ? 2u : (c == 4 ? 1u : 0u); };
+            woff[t] = make_float4(axis_offset(cx, a.g.nx, a.g.cs, a.g.size[0]),
+                                  axis_offset(cy, a.g.ny, a.g.cs, a.g.size[1]),
+                                  axis_offset(cz, a.g.nz, a.g.cs, a.g.size[2]),
+                                  __uint_as_float((img(cx) | (img(cy) << 2) | (img(cz) << 4)) << kIdxBits));
+        }
+        __syncthreads();
+    }
     double e_acc = 0.0;
     double ljx = 0.0, ljy = 0.0, ljz = 0.0;  // this lane's LJ pairs (a.lj)
     float vir[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // triplet virial share of this lane's visits
     unsigned long long nvisit = 0;  // accepted visits of this wave (the same in every lane)
+    unsigned long long npair = 0;   // SKIN: in-range pairs j > i of this wave (the same in every lane)
     bool p_skip = false;
     if (p < a.n) {
-        const int nn = __builtin_amdgcn_readfirstlane((int)a.nlc[p]);
         const uint32_t oi = a.sorted[p];
+        int nn;
+        if (SKIN) {
+            // the in-range neighbours of the Verlet list at the current positions:
+            // r_j - r_i = (q_j + o) - q_i with q the FP32 positions relative to the build-time
+            // cell centres and o the entry's cell offset (the list pass's staging arithmetic,
+            // so a fresh list gives the same vectors); within a few ulps of rc the FP64
+            // distance decides, as in the list pass.  Both row halves are loaded at once
+            // (the row is allocated to its capacity); slots past the length read site p.
+            const float4 qi = a.spos[p];
+            const int nv = __builtin_amdgcn_readfirstlane((int)a.vlc[p]);
+            const uint32_t* vrow = a.vl + (size_t)p * kSkinCap;
+            const uint32_t e0 = vrow[lane], e1 = vrow[64 + lane];
+            int cnt = 0;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int s = h * 64 + lane;
+                const bool live = s < nv;
+                const uint32_t e = live ? (h ? e1 : e0) : (uint32_t)p;
+                const float4 q = a.spos[e & kSkinIdxMask];
+                const float4 o = woff[e >> kSkinIdxBits];
+                float4 d;
+                d.x = (q.x + o.x) - qi.x;
+                d.y = (q.y + o.y) - qi.y;
+                d.z = (q.z + o.z) - qi.z;
+                const uint32_t wj = __float_as_uint(q.w) | __float_as_uint(o.w);
+                d.w = __uint_as_float(wj);
+                const float r2 = live ? d.x * d.x + d.y * d.y + d.z * d.z : __builtin_huge_valf();
+                bool in = r2 <= a.rc2f;
+                const bool amb = fabsf(r2 - a.rc2f) <= a.band;
+                if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, oi, wj) > a.rc2);
+                const unsigned long long bal = ballot(in);
+                npair += (unsigned long long)__popcll(ballot(in && (wj & kIdxMask) > oi));
+                if (in) {
+                    const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    if (slot < kFullCap) wbuf[wvu][slot] = d;
+                }
+                cnt += (int)__popcll(bal);
+            }
+            if (cnt > kFullCap && lane == 0) atomicOr(a.flags, FLAG_NL_OVERFLOW);
+            nn = min(cnt, kFullCap);
+            wave_sync();
+        } else {
+            nn = __builtin_amdgcn_readfirstlane((int)a.nlc[p]);
+        }
         float li = 1.f;
         if (HYB) li = a.lam[oi];
         const float4* row = a.nl + (size_t)p * kFullCap;
         float4 xa = make_float4(1.0e6f * (float)(lane + 1), 0.f, 0.f, 0.f);  // distinct far-away filler
         bool weighted = HYB ? li != 0.f : true;
         for (int s = lane; s < nn; s += 64) {
-            const float4 v = row[s];
-            wbuf[wvu][s] = v;
+            const float4 v = SKIN ? wbuf[wvu][s] : row[s];
+            if (!SKIN) wbuf[wvu][s] = v;
             const float X = v.x * v.x + v.y * v.y + v.z * v.z;
             const float iX = __builtin_amdgcn_rcpf(X);
             wsq[wvu][s] = make_float2(X, iX);
@@ -584,11 +723,16 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
                     const double t = -de * (double)a.dlam[oi];
                     if (a.hyb_axis == 0) dx += t; else if (a.hyb_axis == 1) dy += t; else dz += t;
                 }
-                const double nfx = a.fx[oi] + dx, nfy = a.fy[oi] + dy, nfz = a.fz[oi] + dz;
+                // SKIN: the only force kernel of the step writes the force outright
+                const double nfx = (SKIN ? 0.0 : a.fx[oi]) + dx, nfy = (SKIN ? 0.0 : a.fy[oi]) + dy,
+                             nfz = (SKIN ? 0.0 : a.fz[oi]) + dz;
                 a.fx[oi] = nfx;
                 a.fy[oi] = nfy;
                 a.fz[oi] = nfz;
                 if (a.kick > 0.0) kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
+                if (SKIN && skin_exceeded(a.vx, a.vy, a.vz, a.mass, oi, a.kick, a.dt, nfx, nfy, nfz, a.spos[p],
+                                          a.spb[p], a.lim2))
+                    atomicOr(a.req_next, 1u);
                 if (a.esite) a.esite[oi] = HYB ? de * (double)li : de;
             }
             e_acc = HYB ? de * (double)li : de;
@@ -598,12 +742,18 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
     if (lane == 0) {
         red[wvu] = e_acc;
         red_cnt[wvu] = nvisit;
+        if (SKIN) red_np[wvu] = npair;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
         for (int k = 0; k < kAtmWaves; k++) t += red_cnt[k];
         atomicAdd(&a.counters[b], t);  // no return: the workgroup does not wait on it
+        if (SKIN) {
+            unsigned long long tp = 0;
+            for (int k = 0; k < kAtmWaves; k++) tp += red_np[k];
+            atomicAdd(&a.pair_counters[b], tp);
+        }
         if (ENERGY) {
             double s = 0;
             for (int k = 0; k < kAtmWaves; k++) s += red[k];
@@ -709,7 +859,34 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, do
     return dispatch<false, true>(c, a, lj, nl, energy, lim);
 }
 
-hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite, bool lj) {
+// The Verlet-list pass of a skinned step (parity par): every site within cutoff + skin,
+// run only when the step's rebuild word is set.
+hipError_t launch_skin_list(komd_ctx* c, int par) {
+    auto& d = c->d;
+    CellArgs a{};
+    a.spos = d.spos; a.sprm = d.sprm; a.sid = d.sid; a.start = d.start;
+    a.rx = d.rx; a.ry = d.ry; a.rz = d.rz; a.eps = d.eps; a.sig = d.sig;
+    a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
+    a.vl = d.vl; a.vlc = d.vlc; a.req = d.skin_req + par;
+    a.part = d.red; a.vpart = d.vpart; a.flags = d.flags; a.counters = d.cnt_pair;
+    a.g = c->grid;
+    a.split = c->cell_split;
+    cell_jobs(c, &a.cx, &a.chunk);
+    a.slots = c->xcd_slots;
+    a.nblocks = 8 * a.chunk;
+    const double rl = c->prm.cutoff + c->skin;
+    a.rl2f = (float)(rl * rl);
+    a.rc2 = c->prm.cutoff * c->prm.cutoff;
+    a.rc2f = (float)a.rc2;
+    if (!d.vl || !d.vlc || !d.skin_req || !c->uniform || c->multisite || c->n > (int64_t)kSkinIdxMask)
+        return hipErrorInvalidValue;
+    ScopedTimer tm(c, "cell_nl");
+    k_cell<true, false, false, 3, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite, bool lj,
+                      int skin_par) {
     auto& d = c->d;
     AtmArgs a{};
     a.nl = d.nl; a.nlc = d.nlc; a.nld = d.nld; a.sorted = d.sorted;
@@ -722,6 +899,7 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     a.hyb_axis = c->hyb_axis;
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;
+    a.band = 4e-6f * a.rc2f;
     a.band3 = 1e-5f * a.rc2f;
     a.h16 = c->prm.cutoff >= 1e-2 && c->prm.cutoff <= 1e4;
     a.lj = lj && !energy && !lam;
@@ -730,9 +908,23 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     a.nu = (float)c->prm.nu;
     a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
     a.kick = (!energy && c->kick_in == KICK_IN_ATM) ? 0.5 * c->prm.delta_t : 0.0;
+    a.flags = d.flags;
     ScopedTimer tm(c, "atm");
     const bool hyb = lam != nullptr;
-    if (hyb) {
+    if (skin_par >= 0) {
+        // Verlet-list step: physical ATM with LJ, NVE (fused half-kick), no hybrid
+        if (atm != 1 || energy || hyb || !a.lj || !(a.kick > 0.0) || !d.vl) return hipErrorInvalidValue;
+        if (!d.spb || c->n > (int64_t)kSkinIdxMask) return hipErrorInvalidValue;
+        a.vl = d.vl; a.vlc = d.vlc; a.spos = d.spos; a.spb = d.spb;
+        a.req_next = d.skin_req + (skin_par ^ 1);
+        a.pair_counters = d.cnt_trip + a.nblocks;
+        a.dt = c->prm.delta_t;
+        // rebuild once a site would move further than half the skin, less a margin for the
+        // FP32 distance tests of the list pass
+        const double lim = 0.5 * c->skin - 1e-5 * (c->prm.cutoff + c->skin);
+        a.lim2 = lim > 0 ? lim * lim : 0.0;
+        k_atm<1, false, false, true><<<a.nblocks, 64 * kAtmWaves, 0, c->stream>>>(a);
+    } else if (hyb) {
         if (energy) k_atm<1, true, true><<<a.nblocks, 64 * kAtmWaves, 0, c->stream>>>(a);
         else k_atm<1, false, true><<<a.nblocks, 64 * kAtmWaves, 0, c->stream>>>(a);
     } else if (atm == 2) {

@@ -25,36 +25,39 @@ __device__ __forceinline__ int cell_coord(double com, double low, double cs, int
     return (int)fbin;
 }
 
-// One thread per molecule (a single-site molecule is one site).
-//  [optional] v *= beta; v += f dt/(2m); r += v dt       (rescale + Integrator Step0)
-//  com = sum(r m)/sum(m); wrap r by the COM; com again   (MoleculeContainer.cpp:55-75, LinkedCells.cpp:390-400)
-//  f = 0                                                 (FReset_Kernel LinkedCells.cpp:386-388)
-//  key = flat domain cell of the COM, slot = atomic arrival index
-__global__ __launch_bounds__(kBlock) void k_com_wrap_key(
-    int nmol, const uint32_t* __restrict__ mol_start, double* __restrict__ rx, double* __restrict__ ry,
-    double* __restrict__ rz, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, const double* __restrict__ mass,
-    uint32_t* __restrict__ key, uint32_t* __restrict__ slot, uint32_t* __restrict__ count, Grid g, int bin_site,
-    int drift, double dt, double dt_half, const double* __restrict__ beta_src, uint32_t* __restrict__ flags) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nmol) return;
-    const int s0 = mol_start ? (int)mol_start[m] : m;
-    const int s1 = mol_start ? (int)mol_start[m + 1] : m + 1;
+// Integrator::Step0 of the sites s0..s1-1 (Integrator.cpp:28-32), optionally after the
+// velocity rescale v *= beta of the previous step's thermostat.
+__device__ __forceinline__ void drift_unit(int s0, int s1, double* __restrict__ rx, double* __restrict__ ry,
+                                           double* __restrict__ rz, double* __restrict__ vx,
+                                           double* __restrict__ vy, double* __restrict__ vz,
+                                           const double* __restrict__ fx, const double* __restrict__ fy,
+                                           const double* __restrict__ fz, const double* __restrict__ mass,
+                                           double dt, double dt_half, const double* __restrict__ beta_src) {
     const double beta = beta_src ? beta_src[0] : 1.0;
-    if (drift) {
-        for (int s = s0; s < s1; s++) {
-            double x = vx[s], y = vy[s], z = vz[s];
-            if (beta_src) { x = x * beta; y = y * beta; z = z * beta; }
-            const double k = dt_half / mass[s];
-            x = x + fx[s] * k;
-            y = y + fy[s] * k;
-            z = z + fz[s] * k;
-            vx[s] = x; vy[s] = y; vz[s] = z;
-            rx[s] = rx[s] + x * dt;
-            ry[s] = ry[s] + y * dt;
-            rz[s] = rz[s] + z * dt;
-        }
+    for (int s = s0; s < s1; s++) {
+        double x = vx[s], y = vy[s], z = vz[s];
+        if (beta_src) { x = x * beta; y = y * beta; z = z * beta; }
+        const double k = dt_half / mass[s];
+        x = x + fx[s] * k;
+        y = y + fy[s] * k;
+        z = z + fz[s] * k;
+        vx[s] = x; vy[s] = y; vz[s] = z;
+        rx[s] = rx[s] + x * dt;
+        ry[s] = ry[s] + y * dt;
+        rz[s] = rz[s] + z * dt;
     }
+}
+
+// Container update of one molecule (sites s0..s1-1):
+//  com = sum(r m)/sum(m); wrap r by the COM; com again   (MoleculeContainer.cpp:55-75, LinkedCells.cpp:390-400)
+//  f = 0 when reset_f                                    (FReset_Kernel LinkedCells.cpp:386-388)
+//  key = flat domain cell of the COM, slot = atomic arrival index
+__device__ __forceinline__ void bin_unit(int s0, int s1, double* __restrict__ rx, double* __restrict__ ry,
+                                         double* __restrict__ rz, double* __restrict__ fx, double* __restrict__ fy,
+                                         double* __restrict__ fz, const double* __restrict__ mass,
+                                         uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                         uint32_t* __restrict__ count, const Grid& g, int bin_site, int reset_f,
+                                         uint32_t* __restrict__ flags) {
     double cx, cy, cz;
     if (bin_site) {
         // every site on its own: wrap and bin by its own position (s1 == s0 + 1)
@@ -91,8 +94,6 @@ __global__ __launch_bounds__(kBlock) void k_com_wrap_key(
             }
             cx = ax / tm; cy = ay / tm; cz = az / tm;
         }
-    } else if (!bin_site && s1 - s0 == 1) {
-        // unchanged r: the second COM equals the first bit for bit
     }
     if (!(cx == cx) || !(cy == cy) || !(cz == cz) || cx - cx != 0.0 || cy - cy != 0.0 || cz - cz != 0.0)
         atomicOr(flags, FLAG_NONFINITE);
@@ -104,16 +105,81 @@ __global__ __launch_bounds__(kBlock) void k_com_wrap_key(
     for (int s = s0; s < s1; s++) {
         key[s] = k;
         slot[s] = base + (uint32_t)(s - s0);
-        fx[s] = 0.0; fy[s] = 0.0; fz[s] = 0.0;
+        if (reset_f) { fx[s] = 0.0; fy[s] = 0.0; fz[s] = 0.0; }
     }
+}
+
+// One thread per molecule (a single-site molecule is one site).
+//  [optional] v *= beta; v += f dt/(2m); r += v dt       (rescale + Integrator Step0)
+//  then the container update of bin_unit
+__global__ __launch_bounds__(kBlock) void k_com_wrap_key(
+    int nmol, const uint32_t* __restrict__ mol_start, double* __restrict__ rx, double* __restrict__ ry,
+    double* __restrict__ rz, double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, const double* __restrict__ mass,
+    uint32_t* __restrict__ key, uint32_t* __restrict__ slot, uint32_t* __restrict__ count, Grid g, int bin_site,
+    int drift, int reset_f, double dt, double dt_half, const double* __restrict__ beta_src,
+    uint32_t* __restrict__ flags) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const int s0 = mol_start ? (int)mol_start[m] : m;
+    const int s1 = mol_start ? (int)mol_start[m + 1] : m + 1;
+    if (drift) drift_unit(s0, s1, rx, ry, rz, vx, vy, vz, fx, fy, fz, mass, dt, dt_half, beta_src);
+    bin_unit(s0, s1, rx, ry, rz, fx, fy, fz, mass, key, slot, count, g, bin_site, reset_f, flags);
+}
+
+// Step0 of a Verlet-list run (single-site molecules).  req[0] != 0 marks a rebuild step:
+// then the container update follows as in k_com_wrap_key.  Otherwise the site keeps its
+// build-time cell and
+// no wrap happens (the lists carry the image offsets of the build); only its FP32
+// position relative to that cell's centre is refreshed, as k_sort_repack computes it.
+// Thread 0 clears the request word of the other step parity (the step's last force kernel
+// sets it when some site would move beyond half the skin at the next drift) and counts
+// the rebuilds.  words: [0], [1] the rebuild words of the two parities, [2] rebuilds.
+__global__ __launch_bounds__(kBlock) void k_skin_drift(
+    int n, double* __restrict__ rx, double* __restrict__ ry, double* __restrict__ rz, double* __restrict__ vx,
+    double* __restrict__ vy, double* __restrict__ vz, double* __restrict__ fx, double* __restrict__ fy,
+    double* __restrict__ fz, const double* __restrict__ mass, uint32_t* __restrict__ key,
+    uint32_t* __restrict__ slot, uint32_t* __restrict__ count, Grid g, int bin_site, double dt, double dt_half,
+    uint32_t* __restrict__ words, int par, float4* __restrict__ spos,
+    const uint32_t* __restrict__ inv, uint32_t* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool rebuild = words[par] != 0u;
+    if (i == 0) {
+        words[par ^ 1] = 0u;
+        if (rebuild) words[2] += 1u;
+    }
+    if (i >= n) return;
+    drift_unit(i, i + 1, rx, ry, rz, vx, vy, vz, fx, fy, fz, mass, dt, dt_half, nullptr);
+    if (rebuild) {
+        bin_unit(i, i + 1, rx, ry, rz, fx, fy, fz, mass, key, slot, count, g, bin_site, 1, flags);
+        return;
+    }
+    const double x = rx[i], y = ry[i], z = rz[i];
+    if (!(x - x == 0.0) || !(y - y == 0.0) || !(z - z == 0.0)) atomicOr(flags, FLAG_NONFINITE);
+    const uint32_t k = key[i];
+    const int ix = (int)(k % (uint32_t)g.nx);
+    const int iy = (int)((k / (uint32_t)g.nx) % (uint32_t)g.ny);
+    const int iz = (int)(k / (uint32_t)(g.nx * g.ny));
+    const double ccx = g.low[0] + ((double)ix + 0.5) * g.cs;
+    const double ccy = g.low[1] + ((double)iy + 0.5) * g.cs;
+    const double ccz = g.low[2] + ((double)iz + 0.5) * g.cs;
+    spos[inv[i]] = make_float4((float)(x - ccx), (float)(y - ccy), (float)(z - ccz), __uint_as_float((uint32_t)i));
+}
+
+__global__ void k_skin_arm(uint32_t* __restrict__ req) {
+    if (threadIdx.x == 0) req[0] = 1u;
 }
 
 // Grids above kFuseCells cells: tiles of 1024 cells, coalesced.  k_scan_tiles sums each tile;
 // k_scan_fin offsets a tile by the sum of the tiles before it, scans it in LDS,
 // writes start[] and zeroes the counts.
 constexpr int kScanTile = 1024;
+// req (Verlet-list runs): the step's rebuild word; the binning kernels return at once
+// on steps that keep their lists.
 __global__ __launch_bounds__(kScanTile) void k_scan_tiles(int ncell, const uint32_t* __restrict__ count,
-                                                          uint32_t* __restrict__ tsum) {
+                                                          uint32_t* __restrict__ tsum,
+                                                          const uint32_t* __restrict__ req) {
+    if (req && req[0] == 0u) return;
     __shared__ uint32_t ws[kScanTile / 64];
     const int i = blockIdx.x * kScanTile + threadIdx.x;
     uint32_t v = i < ncell ? count[i] : 0u;
@@ -129,7 +195,9 @@ __global__ __launch_bounds__(kScanTile) void k_scan_tiles(int ncell, const uint3
 
 __global__ __launch_bounds__(kScanTile) void k_scan_fin(int ncell, uint32_t* __restrict__ count,
                                                         uint32_t* __restrict__ start,
-                                                        const uint32_t* __restrict__ tsum) {
+                                                        const uint32_t* __restrict__ tsum,
+                                                        const uint32_t* __restrict__ req) {
+    if (req && req[0] == 0u) return;
     __shared__ uint32_t part[kScanTile];
     __shared__ uint32_t base;
     const int t = threadIdx.x;
@@ -158,7 +226,9 @@ __global__ __launch_bounds__(kScanTile) void k_scan_fin(int ncell, uint32_t* __r
 
 __global__ __launch_bounds__(kBlock) void k_scatter(int n, const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ slot,
-                                                    const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
+                                                    const uint32_t* __restrict__ start, uint32_t* __restrict__ tmp,
+                                                    const uint32_t* __restrict__ req) {
+    if (req && req[0] == 0u) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     tmp[start[key[i]] + slot[i]] = (uint32_t)i;
@@ -171,7 +241,9 @@ constexpr int kFuseCells = 4096;
 __global__ __launch_bounds__(kBlock) void k_scan_scatter(int n, int ncell, const uint32_t* __restrict__ count,
                                                          const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ slot,
-                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ tmp) {
+                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ tmp,
+                                                         const uint32_t* __restrict__ req) {
+    if (req && req[0] == 0u) return;
     __shared__ uint32_t pre[kFuseCells + 1];
     __shared__ uint32_t wsum[kBlock / 64];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -216,7 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_repack(
     uint32_t* __restrict__ sorted, const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ rz, float4* __restrict__ spos, const double* __restrict__ eps,
     const double* __restrict__ sig, float2* __restrict__ sprm, const uint32_t* __restrict__ idcls,
-    uint32_t* __restrict__ sid, uint32_t* __restrict__ inv, Grid g, uint32_t* __restrict__ zero_count) {
+    uint32_t* __restrict__ sid, uint32_t* __restrict__ inv, Grid g, uint32_t* __restrict__ zero_count,
+    const uint32_t* __restrict__ req, float4* __restrict__ spb) {
+    if (req && req[0] == 0u) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (zero_count && i < g.ncell) zero_count[i] = 0;  // histogram of the next binning
     if (i >= n) return;
@@ -233,8 +307,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_repack(
     const double ccx = g.low[0] + ((double)ix + 0.5) * g.cs;
     const double ccy = g.low[1] + ((double)iy + 0.5) * g.cs;
     const double ccz = g.low[2] + ((double)iz + 0.5) * g.cs;
-    spos[p] = make_float4((float)(rx[i] - ccx), (float)(ry[i] - ccy), (float)(rz[i] - ccz),
-                          __uint_as_float((uint32_t)i));
+    const float4 q = make_float4((float)(rx[i] - ccx), (float)(ry[i] - ccy), (float)(rz[i] - ccz),
+                                 __uint_as_float((uint32_t)i));
+    spos[p] = q;
+    if (spb) spb[p] = q;  // Verlet-list runs: the position of the rebuild
     if (sprm) sprm[p] = make_float2((float)sig[i], (float)sqrt(eps[i]));
     if (sid) sid[p] = idcls[i];
 }
@@ -386,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double* __restric
 
 }  // namespace
 
-hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift) {
+hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift, bool reset_f) {
     ScopedTimer tm(c, "bin_key");
     auto& d = c->d;
     const bool site = c->prm.bin_mode == KOMD_BIN_SITE;
@@ -394,7 +470,7 @@ hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift) {
     const uint32_t* ms = (site || !c->multisite) ? nullptr : d.mol_start;
     k_com_wrap_key<<<(units + kKeyBlock - 1) / kKeyBlock, kKeyBlock, 0, c->stream>>>(
         units, ms, d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.key, d.slot, d.count, c->grid,
-        site ? 1 : 0, do_drift ? 1 : 0, c->prm.delta_t, c->prm.delta_t * 0.5, nullptr, d.flags);
+        site ? 1 : 0, do_drift ? 1 : 0, reset_f ? 1 : 0, c->prm.delta_t, c->prm.delta_t * 0.5, nullptr, d.flags);
     return hipGetLastError();
 }
 
@@ -407,11 +483,27 @@ hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev) {
     const uint32_t* ms = (site || !c->multisite) ? nullptr : d.mol_start;
     k_com_wrap_key<<<(units + kKeyBlock - 1) / kKeyBlock, kKeyBlock, 0, c->stream>>>(
         units, ms, d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.key, d.slot, d.count, c->grid,
-        site ? 1 : 0, 1, c->prm.delta_t, c->prm.delta_t * 0.5, beta_dev, d.flags);
+        site ? 1 : 0, 1, 1, c->prm.delta_t, c->prm.delta_t * 0.5, beta_dev, d.flags);
     return hipGetLastError();
 }
 
-hipError_t launch_bin(komd_ctx* c) {
+hipError_t launch_skin_drift(komd_ctx* c, int par) {
+    ScopedTimer tm(c, "bin_key");
+    auto& d = c->d;
+    const int n = (int)c->n;
+    k_skin_drift<<<(n + kKeyBlock - 1) / kKeyBlock, kKeyBlock, 0, c->stream>>>(
+        n, d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.key, d.slot, d.count, c->grid,
+        c->prm.bin_mode == KOMD_BIN_SITE ? 1 : 0, c->prm.delta_t, c->prm.delta_t * 0.5, d.skin_req, par, d.spos,
+        d.inv, d.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_skin_arm(komd_ctx* c, int par) {
+    k_skin_arm<<<1, 64, 0, c->stream>>>(c->d.skin_req + par);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin(komd_ctx* c, const uint32_t* req, float4* spb) {
     auto& d = c->d;
     const int n = (int)c->n;
     const int C = c->grid.ncell;
@@ -419,19 +511,19 @@ hipError_t launch_bin(komd_ctx* c) {
     if (fused) {
         ScopedTimer tm(c, "bin_scatter");
         k_scan_scatter<<<std::max(1, (n + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(n, C, d.count, d.key, d.slot,
-                                                                                       d.start, d.tmp);
+                                                                                       d.start, d.tmp, req);
         KOMD_CHECK_HIP(hipGetLastError());
     } else {
         {
             ScopedTimer tm(c, "bin_scan");
             const int tiles = (C + kScanTile - 1) / kScanTile;
-            k_scan_tiles<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.scanp);
+            k_scan_tiles<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.scanp, req);
             KOMD_CHECK_HIP(hipGetLastError());
-            k_scan_fin<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.start, d.scanp);
+            k_scan_fin<<<tiles, kScanTile, 0, c->stream>>>(C, d.count, d.start, d.scanp, req);
             KOMD_CHECK_HIP(hipGetLastError());
         }
         ScopedTimer tm(c, "bin_scatter");
-        k_scatter<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(n, d.key, d.slot, d.start, d.tmp);
+        k_scatter<<<(n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(n, d.key, d.slot, d.start, d.tmp, req);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     {
@@ -439,7 +531,7 @@ hipError_t launch_bin(komd_ctx* c) {
         const int units = std::max(n, fused ? C : 0);
         k_sort_repack<<<(units + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
             n, d.key, d.start, d.tmp, d.sorted, d.rx, d.ry, d.rz, d.spos, d.eps, d.sig, c->uniform ? nullptr : d.sprm,
-            d.idcls, c->multisite ? d.sid : nullptr, d.inv, c->grid, fused ? d.count : nullptr);
+            d.idcls, c->multisite ? d.sid : nullptr, d.inv, c->grid, fused ? d.count : nullptr, req, spb);
         KOMD_CHECK_HIP(hipGetLastError());
     }
     return hipSuccess;

@@ -67,7 +67,7 @@ void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
                     d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.scanp, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
-                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf};
+                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf, d.vl, d.vlc, d.spb, d.skin_req};
     for (void* p : ptrs)
         if (p) hipFree(p);
     DevState keep;
@@ -85,7 +85,9 @@ void free_sites(komd_ctx* c) {
 
 void drop_graph(komd_ctx* c) {
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph_exec_odd) hipGraphExecDestroy(c->graph_exec_odd);
     c->graph_exec = nullptr;
+    c->graph_exec_odd = nullptr;
     c->graph_key = -1;
 }
 
@@ -119,6 +121,10 @@ komd_status sync_check(komd_ctx* c, const char* where) {
             return fail(c, KOMD_ERR_OVERFLOW,
                         std::string(where) + ": a site has more than " + std::to_string(kFullCap) +
                             " in-range neighbours (the neighbour-list capacity)");
+        if (flags & FLAG_VL_OVERFLOW)
+            return fail(c, KOMD_ERR_OVERFLOW,
+                        std::string(where) + ": a site has more than " + std::to_string(kSkinCap) +
+                            " neighbours within cutoff + skin (the Verlet-list capacity; use a smaller skin)");
         if (flags & FLAG_CELL_OVERFLOW)
             return fail(c, KOMD_ERR_OVERFLOW,
                         std::string(where) + ": surrogate force pass: a cell neighbourhood holds more than " +
@@ -130,18 +136,25 @@ komd_status sync_check(komd_ctx* c, const char* where) {
     return KOMD_OK;
 }
 
-// Sums the per-workgroup count slots: [0] unique in-range pairs, [1] triplet visits.
+// Sums the per-workgroup count slots: [0] unique in-range pairs (the pair pass's slots plus
+// those of the Verlet-list ATM pass, cnt_trip[atm_blocks ..]), [1] triplet visits.
 komd_status read_counters(komd_ctx* c, int64_t out[2], bool reset) {
     HIPCALL(c, hipStreamSynchronize(c->stream), "komd_counters");
-    const int nb[2] = {cell_blocks(c), atm_blocks(c)};
-    unsigned long long* src[2] = {c->d.cnt_pair, c->d.cnt_trip};
-    for (int k = 0; k < 2; k++) {
-        std::vector<unsigned long long> h((size_t)nb[k]);
-        HIPCALL(c, hipMemcpy(h.data(), src[k], h.size() * 8, hipMemcpyDeviceToHost), "komd_counters");
-        unsigned long long t = 0;
-        for (auto x : h) t += x;
-        out[k] = (int64_t)t;
-        if (reset) HIPCALL(c, hipMemset(src[k], 0, h.size() * 8), "komd_counters");
+    const int na = atm_blocks(c);
+    std::vector<unsigned long long> hp((size_t)cell_blocks(c)), ht((size_t)2 * na);
+    HIPCALL(c, hipMemcpy(hp.data(), c->d.cnt_pair, hp.size() * 8, hipMemcpyDeviceToHost), "komd_counters");
+    HIPCALL(c, hipMemcpy(ht.data(), c->d.cnt_trip, ht.size() * 8, hipMemcpyDeviceToHost), "komd_counters");
+    unsigned long long pairs = 0, trip = 0;
+    for (auto x : hp) pairs += x;
+    for (int k = 0; k < na; k++) {
+        trip += ht[(size_t)k];
+        pairs += ht[(size_t)(na + k)];
+    }
+    out[0] = (int64_t)pairs;
+    out[1] = (int64_t)trip;
+    if (reset) {
+        HIPCALL(c, hipMemset(c->d.cnt_pair, 0, hp.size() * 8), "komd_counters");
+        HIPCALL(c, hipMemset(c->d.cnt_trip, 0, ht.size() * 8), "komd_counters");
     }
     return KOMD_OK;
 }
@@ -161,6 +174,19 @@ hipError_t ensure_nl(komd_ctx* c, bool dirs) {
     if (!d.nlc) KOMD_CHECK_HIP(dalloc(&d.nlc, (size_t)c->n));
     if (dirs && !d.nld) KOMD_CHECK_HIP(dalloc(&d.nld, (size_t)c->n * kFullCap));
     if (dirs && !d.rows) KOMD_CHECK_HIP(dalloc(&d.rows, (size_t)c->grid.ncell * 27 * kRows));
+    return hipSuccess;
+}
+
+// Verlet-list buffers, allocated on the first skinned run.
+hipError_t ensure_skin(komd_ctx* c) {
+    auto& d = c->d;
+    if (!d.vl) KOMD_CHECK_HIP(dalloc(&d.vl, (size_t)c->n * kSkinCap));
+    if (!d.vlc) KOMD_CHECK_HIP(dalloc(&d.vlc, (size_t)c->n));
+    if (!d.spb) KOMD_CHECK_HIP(dalloc(&d.spb, (size_t)c->n));
+    if (!d.skin_req) {
+        KOMD_CHECK_HIP(dalloc(&d.skin_req, 4));
+        KOMD_CHECK_HIP(hipMemset(d.skin_req, 0, 4 * sizeof(uint32_t)));
+    }
     return hipSuccess;
 }
 }  // namespace
@@ -253,6 +279,27 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     }
     if (kind3b == KOMD_3B_HYBRID) return hybrid_forces(c, false);
     return hipSuccess;
+}
+
+// Runs that use Verlet lists (komd_set_skin): the exact ATM with LJ summed in the three-body
+// kernel (uniform single-site sites, LJ 12-6, no clamp), NVE, no bonded terms.
+bool skin_run(const komd_ctx* c, int kind3b, bool bonded, bool thermo) {
+    return c->skin > 0 && c->n <= (int64_t)kSkinIdxMask && kind3b == KOMD_3B_ATM_EXACT && !bonded && !thermo && c->pair_kind == KOMD_PAIR_LJ12_6 &&
+           c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+}
+
+// One step with Verlet lists, parity par (Simulation.cpp:49-74 with the pair list of
+// LinkedCells::updatePairList rebuilt only when needed, PairList.h:22-24): Step0 and the
+// refreshed FP32 positions; on a rebuild step the container update, binning and the list
+// pass (the other kernels return at once); LJ + ATM from the lists with Step1 fused.
+hipError_t enqueue_skin_step(komd_ctx* c, int par) {
+    KOMD_CHECK_HIP(launch_skin_drift(c, par));
+    KOMD_CHECK_HIP(launch_bin(c, c->d.skin_req + par, c->d.spb));
+    KOMD_CHECK_HIP(launch_skin_list(c, par));
+    c->kick_in = KICK_IN_ATM;
+    const hipError_t e = launch_atm(c, 1, false, nullptr, nullptr, nullptr, true, par);
+    c->kick_in = KICK_NONE;
+    return e;
 }
 
 // One reference-order step (Simulation.cpp:49-74) queued on the stream.
@@ -454,9 +501,10 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
         e = dalloc(&d.vpart, (size_t)kSym * std::max<size_t>({(size_t)cell_blocks(c) + 1, (size_t)atm_blocks(c),
                                                                (size_t)512, (size_t)kKinBlocks}));
     if (e == hipSuccess) e = dalloc(&d.cnt_pair, (size_t)cell_blocks(c));
-    if (e == hipSuccess) e = dalloc(&d.cnt_trip, (size_t)atm_blocks(c));
+    // triplet visits, then the in-range pairs of Verlet-list ATM passes
+    if (e == hipSuccess) e = dalloc(&d.cnt_trip, (size_t)2 * atm_blocks(c));
     if (e == hipSuccess) e = hipMemset(d.cnt_pair, 0, (size_t)cell_blocks(c) * 8);
-    if (e == hipSuccess) e = hipMemset(d.cnt_trip, 0, (size_t)atm_blocks(c) * 8);
+    if (e == hipSuccess) e = hipMemset(d.cnt_trip, 0, (size_t)2 * atm_blocks(c) * 8);
     if (e != hipSuccess) {
         free_sites(c);
         return fail(c, KOMD_ERR_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
@@ -743,10 +791,86 @@ komd_status komd_compute_forces(komd_ctx* c, int32_t kind3b, int32_t use_bonded)
 }  // extern "C"
 
 namespace {
+// Error flags copied to a pinned host slot every kFlagEvery steps; the host reads the previous
+// copy (kFlagEvery steps older) when it queues the next one, so an overflow or a non-finite
+// state stops the run within 2 kFlagEvery steps without draining the queue at every check.
+constexpr int64_t kFlagEvery = 32;
+
+struct FlagWatch {
+    bool pending[2] = {false, false};
+    // after step k (0-based) of nsteps has been queued
+    komd_status after(komd_ctx* c, int64_t k, int64_t nsteps) {
+        if ((k + 1) % kFlagEvery != 0 || k + 1 >= nsteps) return KOMD_OK;
+        const int slot = (int)(((k + 1) / kFlagEvery) & 1), prev = slot ^ 1;
+        if (pending[prev]) {
+            HIPCALL(c, hipEventSynchronize(c->flag_ev[prev]), "komd_run");
+            pending[prev] = false;
+            if (c->flag_host[prev]) {
+                komd_status s = sync_check(c, "komd_run");
+                if (s) return s;
+            }
+        }
+        HIPCALL(c, launch_flag_snapshot(c, c->flag_host + slot), "komd_run");
+        HIPCALL(c, hipEventRecord(c->flag_ev[slot], c->stream), "komd_run");
+        pending[slot] = true;
+        return KOMD_OK;
+    }
+};
+
+// komd_run with Verlet lists (skin_run): the first step always rebuilds; a step's last
+// kernel arms the next step's rebuild when some site would leave half the skin.  The step
+// parities alternate between two captured graphs (each reads its own rebuild word and
+// clears the other's).  The run ends with a container update without force reset, so the
+// positions are wrapped into the box and the grid matches them, as after a plain run.
+komd_status run_skin(komd_ctx* c, int64_t nsteps) {
+    HIPCALL(c, ensure_skin(c), "komd_run");
+    HIPCALL(c, launch_skin_arm(c, c->skin_par), "komd_run");
+    if (c->timing || nsteps < 3) {
+        for (int64_t k = 0; k < nsteps; k++) {
+            HIPCALL(c, enqueue_skin_step(c, c->skin_par), "komd_run");
+            c->skin_par ^= 1;
+            if (c->timing && (k % 64) == 63) {
+                komd_status s = sync_check(c, "komd_run");
+                if (s) return s;
+            }
+        }
+    } else {
+        const int key = 1 << 20;
+        if (!c->graph_exec || !c->graph_exec_odd || c->graph_key != key) {
+            drop_graph(c);
+            for (int par = 0; par < 2; par++) {
+                hipGraph_t graph = nullptr;
+                HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "komd_run capture");
+                hipError_t e = enqueue_skin_step(c, par);
+                hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+                HIPCALL(c, e, "komd_run capture");
+                HIPCALL(c, e2, "komd_run capture");
+                e = hipGraphInstantiate(par ? &c->graph_exec_odd : &c->graph_exec, graph, nullptr, nullptr, 0);
+                hipGraphDestroy(graph);
+                HIPCALL(c, e, "komd_run instantiate");
+            }
+            c->graph_key = key;
+        }
+        FlagWatch fw;
+        for (int64_t k = 0; k < nsteps; k++) {
+            HIPCALL(c, hipGraphLaunch(c->skin_par ? c->graph_exec_odd : c->graph_exec, c->stream), "komd_run launch");
+            c->skin_par ^= 1;
+            komd_status s = fw.after(c, k, nsteps);
+            if (s) return s;
+        }
+    }
+    if (nsteps > 0) {
+        HIPCALL(c, launch_com_wrap_key(c, false, false), "komd_run");
+        HIPCALL(c, launch_bin(c), "komd_run");
+    }
+    return sync_check(c, "komd_run");
+}
+
 komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bonded, double t0) {
     komd_status s = KOMD_OK;
     const bool bonded = use_bonded && (c->nbonds + c->nangles) > 0;
     const bool thermo = t0 > 0;
+    if (skin_run(c, kind3b, bonded, thermo)) return run_skin(c, nsteps);
     // beta = 1 for the first kick-drift and the target the captured step reads, stream-ordered
     // ahead of the steps (the graph holds no host value of t0, so one graph serves every target)
     if (thermo) HIPCALL(c, launch_run_scalars(c, t0), "komd_run");
@@ -789,28 +913,11 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
             HIPCALL(c, e, "komd_run instantiate");
             c->graph_key = key;
         }
-        // Every kFlagEvery steps the error flags are copied to a pinned host slot; the host
-        // reads the previous copy (kFlagEvery steps older) when it queues the next one, so an
-        // overflow or a non-finite state stops the run within 2 kFlagEvery steps without
-        // draining the queue at every check.
-        constexpr int64_t kFlagEvery = 32;
-        bool pending[2] = {false, false};
+        FlagWatch fw;
         for (int64_t k = 0; k < nsteps; k++) {
             HIPCALL(c, hipGraphLaunch(c->graph_exec, c->stream), "komd_run launch");
-            if ((k + 1) % kFlagEvery == 0 && k + 1 < nsteps) {
-                const int slot = (int)(((k + 1) / kFlagEvery) & 1), prev = slot ^ 1;
-                if (pending[prev]) {
-                    HIPCALL(c, hipEventSynchronize(c->flag_ev[prev]), "komd_run");
-                    pending[prev] = false;
-                    if (c->flag_host[prev]) {
-                        s = sync_check(c, "komd_run");
-                        if (s) return s;
-                    }
-                }
-                HIPCALL(c, launch_flag_snapshot(c, c->flag_host + slot), "komd_run");
-                HIPCALL(c, hipEventRecord(c->flag_ev[slot], c->stream), "komd_run");
-                pending[slot] = true;
-            }
+            s = fw.after(c, k, nsteps);
+            if (s) return s;
         }
     }
     if (thermo && nsteps > 0) HIPCALL(c, launch_rescale(c, t0), "komd_run");
@@ -856,6 +963,32 @@ komd_status komd_run(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_bo
             return KOMD_OK;
         }
     }
+    return KOMD_OK;
+}
+
+komd_status komd_set_skin(komd_ctx* c, double skin) {
+    if (!c) return KOMD_ERR_ARG;
+    if (!(skin >= 0) || !std::isfinite(skin)) return fail(c, KOMD_ERR_ARG, "skin must be finite and >= 0");
+    if (skin > 0 && c->have_sites && c->n > (int64_t)kSkinIdxMask)
+        return fail(c, KOMD_ERR_ARG, "skin: Verlet lists hold at most 2^25 - 1 sites");
+    if (skin > 0 && c->grid.cs < c->prm.cutoff + skin)
+        return fail(c, KOMD_ERR_ARG,
+                    "skin: the cell edge (" + std::to_string(c->grid.cs) + ") must be at least cutoff + skin (" +
+                        std::to_string(c->prm.cutoff + skin) + ")");
+    c->skin = skin;
+    drop_graph(c);
+    if (c->d.skin_req) HIPCALL(c, hipMemset(c->d.skin_req + 2, 0, sizeof(uint32_t)), "komd_set_skin");
+    return KOMD_OK;
+}
+
+komd_status komd_skin_rebuilds(komd_ctx* c, int64_t* rebuilds) {
+    if (!c || !rebuilds) return KOMD_ERR_ARG;
+    *rebuilds = 0;
+    if (!c->d.skin_req) return KOMD_OK;
+    uint32_t h = 0;
+    HIPCALL(c, hipStreamSynchronize(c->stream), "komd_skin_rebuilds");
+    HIPCALL(c, hipMemcpy(&h, c->d.skin_req + 2, sizeof(uint32_t), hipMemcpyDeviceToHost), "komd_skin_rebuilds");
+    *rebuilds = (int64_t)h;
     return KOMD_OK;
 }
 

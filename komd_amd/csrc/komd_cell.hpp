@@ -11,6 +11,11 @@ constexpr int kMaxM = 768;         // staged neighbourhood capacity when the thr
 constexpr int kChunkLJ = 768;      // staging chunk of the pair-only pass (any neighbourhood size)
 constexpr int kMaxOwn = 64;        // home sites per cell when the three-body pass runs
 constexpr int kFullCap = 96;       // full neighbour list capacity per home site (three-body pass)
+constexpr int kSkinCap = 128;      // Verlet list capacity per site (within cutoff + skin)
+// Verlet-list entry: cell-sorted slot of the neighbour (25 bits, so at most 2^25 - 1 sites)
+// | the offset class of its cell (0..124) << 25
+constexpr uint32_t kSkinIdxBits = 25;
+constexpr uint32_t kSkinIdxMask = (1u << kSkinIdxBits) - 1u;
 constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row
 constexpr int kSurM = 640;         // staged neighbourhood capacity of the surrogate force pass
 

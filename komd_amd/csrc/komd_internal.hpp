@@ -15,6 +15,9 @@
 //                  (r_j - r_i) in FP32, .w = j | image code << 26; nlc[N] its length,
 //                  nld[N][96] u8 neighbour cell direction (reference-mode ATM only)
 //   esite[N] f64   per-site three-body energy share (training targets)
+//   vl[N][128] u32 Verlet list of skinned runs: cell-sorted slot of every site within
+//                  cutoff + skin at the last rebuild | the offset class of its cell << 25;
+//                  vlc[N] its length, spb[N] float4 the FP32 positions of that rebuild
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -31,7 +34,7 @@ constexpr uint32_t kIdxBits = 26;
 constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
 
 // error flags raised by kernels (bitmask in dev->flags)
-enum : uint32_t { FLAG_NL_OVERFLOW = 1u, FLAG_NONFINITE = 2u, FLAG_CELL_OVERFLOW = 4u };
+enum : uint32_t { FLAG_NL_OVERFLOW = 1u, FLAG_NONFINITE = 2u, FLAG_CELL_OVERFLOW = 4u, FLAG_VL_OVERFLOW = 8u };
 
 // Slots of the device scalars (DevState::scal, 32 doubles).
 enum : int {
@@ -78,6 +81,12 @@ struct DevState {
     float4* nl = nullptr;
     uint32_t* nlc = nullptr;
     uint16_t* nld = nullptr;      // [N][kFullCap] neighbour cell direction | rank in that cell << 5
+    // Verlet lists of skinned runs (komd_set_skin): every site within cutoff + skin at the
+    // last rebuild, entry = cell-sorted slot of the neighbour | offset class of its cell << 25
+    uint32_t* vl = nullptr;       // [N][kSkinCap]
+    uint32_t* vlc = nullptr;      // [N] lengths
+    float4* spb = nullptr;        // [N] cell-sorted FP32 positions of the last rebuild
+    uint32_t* skin_req = nullptr; // [0], [1] rebuild words, one per step parity; [2] rebuilds so far
     // bonded
     uint32_t *bonds = nullptr, *angles = nullptr;
     uint32_t *bond_mol = nullptr, *angle_mol = nullptr;  // CSR of bonds / angles per molecule
@@ -132,9 +141,14 @@ struct komd_ctx {
     std::unordered_map<std::string, komd::Timer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
-    // graph of one step
+    // graph of one step (skinned runs: graph_exec for even steps, graph_exec_odd for odd ones)
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec_odd = nullptr;
     int graph_key = -1;
+    // Verlet lists (komd_set_skin): skin > 0 enables them for the runs that can use them;
+    // skin_par is the parity of the next queued step
+    double skin = 0.0;
+    int skin_par = 0;
     // surrogate
     komd::Surrogate* sur = nullptr;
     // pair force of komd_run / komd_compute_forces / komd_measure_observables
@@ -180,8 +194,14 @@ struct ScopedTimer {
 
 // ---- launch wrappers --------------------------------------------------
 // k_container.hip
-hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift);
-hipError_t launch_bin(komd_ctx* c);             // scan + scatter + stable sort + FP32 repack
+hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift, bool reset_f = true);
+// scan + scatter + stable sort + FP32 repack; req (Verlet-list runs): the step's rebuild
+// word, the kernels return at once when it is zero; spb: a copy of the FP32 positions
+hipError_t launch_bin(komd_ctx* c, const uint32_t* req = nullptr, float4* spb = nullptr);
+// Verlet-list runs: Step0 + (rebuild steps) container update, else the FP32 position
+// refresh; par is the step parity whose rebuild word is read
+hipError_t launch_skin_drift(komd_ctx* c, int par);
+hipError_t launch_skin_arm(komd_ctx* c, int par);  // rebuild word of parity par = 1
 hipError_t launch_kick_drift(komd_ctx* c);      // integrate0
 hipError_t launch_kick(komd_ctx* c, bool with_temp);  // integrate1 (+ kinetic partials)
 // v += f (dt/2)/m in the reference's FP64 operation order (Integrator.cpp:34-37).
@@ -223,8 +243,10 @@ enum { KICK_NONE = 0, KICK_IN_CELL = 1, KICK_IN_ATM = 2, KICK_IN_APPLY = 3, KICK
 hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, double scale = 1.0);
 // lj: the kernel also adds the LJ force of every listed pair (uniform single-site
 // systems without the force clamp; the pair pass then skips LJ)
+// skin_par >= 0: a Verlet-list step of that parity (launch_skin_list built or kept the lists)
 hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite,
-                      bool lj = false);
+                      bool lj = false, int skin_par = -1);
+hipError_t launch_skin_list(komd_ctx* c, int par);
 int cell_blocks(komd_ctx* c);
 int atm_blocks(komd_ctx* c);
 // k_table.hip: tabulated pair force over current neighbour lists; with energy the pair

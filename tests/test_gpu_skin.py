@@ -1,0 +1,189 @@
+"""Verlet-list (skinned) runs of the exact three-body argon path, through the C ABI.
+
+The reference reuses its pair list for pair_steps steps without a skin
+(PairList.h:22-24, LinkedCells.cpp:130-131), so its stale steps miss pairs; the
+skinned run keeps every site within cutoff + skin at its last rebuild, tests the
+cutoff at the current positions every step and rebuilds before any site has moved
+half the skin.  Bars:
+  * forces after steps on a list built several steps earlier equal the FP64
+    oracle's LJ (C08 walk) + ATM at the same positions: max 2e-4, RMS 5e-6 of the
+    RMS force (the bars of the one-step force tests);
+  * the unique in-range pairs and triplet visits summed over a run equal those of
+    a run that rebuilds the container every step from the same state, and the
+    trajectories agree to 1e-9 (only FP32/FP64 summation orders differ);
+  * a list over its capacity is an error, runs the lists cannot serve ignore the
+    skin (bit-identical to a context without one).
+"""
+import numpy as np
+import pytest
+
+import komd_amd as K
+from komd_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.073
+SKIN = 0.3
+
+
+def rel_err(a, b):
+    scale = np.sqrt(np.mean(b * b))
+    d = np.linalg.norm(a - b, axis=1)
+    return float(d.max() / scale), float(np.sqrt(np.mean(d * d)) / scale)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle
+    oracle.build()
+    return oracle
+
+
+def argon(cells, seed, skin=SKIN, rho=0.8, T=0.8):
+    """FCC argon whose cells have an edge of at least cutoff + skin."""
+    return S.fcc(cells, rho, T, seed, rc=2.5 + skin)
+
+
+def oracle_forces(orc, s, r):
+    t = s.copy()
+    t.r = r
+    rr, b = orc.update_container(t)
+    f_lj = orc.lj_c08(t, rr, b, nthreads=8)
+    f_atm = orc.atm_complete(t, rr, b, NU, coverage=0, sign=-1.0, nthreads=8)[0]
+    return f_lj + f_atm
+
+
+@pytest.mark.parametrize("cells,seed,steps", [(10, 61, 6), (10, 62, 60), (20, 63, 40)])
+def test_skin_forces_match_oracle(orc, cells, seed, steps):
+    s = argon(cells, seed)
+    print(f"skin forces n={s.n} steps={steps}", flush=True)
+    with K.Context(s, nu=NU) as c:
+        c.set_skin(SKIN)
+        c.compute_forces(K.ATM_EXACT)
+        c.run(steps, kind3b=K.ATM_EXACT)
+        reb = c.skin_rebuilds()
+        r, _, f = c.get_state()
+    assert np.all((r >= s.low) & (r <= s.high)), "the run must end with wrapped positions"
+    f_or = oracle_forces(orc, s, r)
+    mx, rms = rel_err(f, f_or)
+    print(f"  rebuilds {reb}; forces vs oracle max {mx:.3e} rms {rms:.3e}")
+    assert 1 <= reb < steps
+    if steps == 6:
+        assert reb == 1, "the case needs its last step on the list of the first"
+    assert mx < 2e-4 and rms < 5e-6
+
+
+def fp64_run(orc, s, steps, dt=0.005):
+    """The FP64 trajectory of the oracle: vv_step0 -> container update -> LJ (C08 walk) +
+    ATM over every triplet -> vv_step1 (the order of Simulation.cpp:49-74)."""
+    def forces(r):
+        rb, b = orc.update_container(s, r)
+        f = orc.lj_c08(s, rb, b, nthreads=8)
+        return rb, f + orc.atm_complete(s, rb, b, NU, coverage=0, sign=-1.0, nthreads=8)[0]
+
+    r, f = forces(s.r)
+    v = s.v.copy()
+    for _ in range(steps):
+        r, v = orc.vv_step0(dt, r, v, f, s.mass)
+        r, f = forces(r)
+        v = orc.vv_step1(dt, v, f, s.mass)
+    return r, v
+
+
+def min_image(d, box):
+    return d - box * np.round(d / box)
+
+
+@pytest.mark.parametrize("skin,steps", [(SKIN, 80), (0.02, 30)])
+def test_skin_run_follows_plain_run(orc, skin, steps):
+    """skin 0.02: a rebuild nearly every step (the rebuild path itself); 0.3: rare.
+    Both GPU runs round positions to FP32 relative to a cell centre (the per-step run to
+    the current cell's, the skinned run to the build-time cell's), so their trajectories
+    part at the FP32 level; the skinned run must track the FP64 trajectory as closely as
+    the per-step run does, with the same interaction set every step."""
+    s = argon(10, 64, skin=skin)
+    print(f"skin vs plain n={s.n} skin={skin} steps={steps}", flush=True)
+    with K.Context(s, nu=NU) as a, K.Context(s, nu=NU) as b:
+        b.set_skin(skin)
+        out = []
+        for c in (a, b):
+            c.compute_forces(K.ATM_EXACT)
+            c.counters(reset=True)
+            c.run(steps, kind3b=K.ATM_EXACT)
+            out.append((c.get_state(), c.counters()))
+        reb = b.skin_rebuilds()
+    (ra, va, _), ca = out[0]
+    (rb, vb, _), cb = out[1]
+    r64, v64 = fp64_run(orc, s, steps)
+    box = s.box
+    ea = float(np.abs(min_image(ra - r64, box)).max())
+    eb = float(np.abs(min_image(rb - r64, box)).max())
+    eva, evb = float(np.abs(va - v64).max()), float(np.abs(vb - v64).max())
+    print(f"  rebuilds {reb}; vs FP64: per-step |dr| {ea:.2e} |dv| {eva:.2e}, skinned |dr| {eb:.2e} "
+          f"|dv| {evb:.2e}; counts {ca} {cb}")
+    assert ca == cb, "the same in-range pairs and triplets every step"
+    assert eb < 3.0 * ea + 1e-9 and evb < 3.0 * eva + 1e-8
+    assert eb < 1e-5 and evb < 1e-4
+    if skin < 0.1:
+        assert reb > steps // 2
+    else:
+        assert reb < steps // 4
+
+
+def test_skin_many_rebuild_segments_in_one_run(orc):
+    """A long run at T* 1.5 crosses many rebuilds; its end state still has exact forces."""
+    s = argon(10, 65, T=1.5)
+    with K.Context(s, nu=NU) as c:
+        c.set_skin(SKIN)
+        c.compute_forces(K.ATM_EXACT)
+        c.run(300, kind3b=K.ATM_EXACT)
+        reb = c.skin_rebuilds()
+        r, _, f = c.get_state()
+    mx, rms = rel_err(f, oracle_forces(orc, s, r))
+    print(f"skin 300 steps T1.5: rebuilds {reb}; forces max {mx:.3e} rms {rms:.3e}")
+    assert 10 <= reb < 150
+    assert mx < 2e-4 and rms < 5e-6
+
+
+def test_skin_list_overflow_is_an_error():
+    """Density 1.1 with cutoff + skin = 3.1: about 136 sites within the list radius, over the
+    list capacity of 128."""
+    s = S.fcc(8, 1.1, 0.8, 66, rc=3.1)
+    with K.Context(s, nu=NU) as c:
+        c.set_skin(0.6)
+        c.compute_forces(K.ATM_EXACT)
+        with pytest.raises(K.KomdError) as ei:
+            c.run(5, kind3b=K.ATM_EXACT)
+    assert ei.value.status == 4 and "Verlet-list capacity" in str(ei.value)
+
+
+def test_skin_arguments():
+    s = argon(10, 67)
+    with K.Context(s, nu=NU) as c:
+        for bad in (-0.1, float("nan"), float("inf")):
+            with pytest.raises(K.KomdError) as ei:
+                c.set_skin(bad)
+            assert ei.value.status == 1
+        # the cells of this system are 2.8 + a little: a skin of 0.5 needs 3.0
+        with pytest.raises(K.KomdError) as ei:
+            c.set_skin(0.5)
+        assert ei.value.status == 1 and "cell edge" in str(ei.value)
+        c.set_skin(0.3)
+        c.set_skin(0.0)
+
+
+def test_skin_is_ignored_where_it_does_not_apply():
+    """A thermostatted run and the surrogate-free LJ-only run keep their per-step container
+    update: bit-identical to a context without a skin."""
+    s = argon(10, 68)
+    with K.Context(s, nu=NU) as a, K.Context(s, nu=NU) as b:
+        b.set_skin(SKIN)
+        res = []
+        for c in (a, b):
+            c.compute_forces(K.ATM_EXACT)
+            c.run(20, kind3b=K.ATM_EXACT, t0=0.9)
+            c.run(20, kind3b=0)
+            res.append(c.get_state())
+        assert b.skin_rebuilds() == 0
+    for x, y in zip(res[0], res[1]):
+        assert np.array_equal(x, y)
