@@ -393,6 +393,9 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+}  // namespace
+struct SkinArgs;
+namespace {
 struct AtmArgs {
     const float4* __restrict__ nl;
     const uint32_t* __restrict__ nlc;
@@ -414,16 +417,10 @@ struct AtmArgs {
     double* __restrict__ vy;
     double* __restrict__ vz;
     const double* __restrict__ mass;
-    // Verlet-list steps (SKIN): the list, the FP32 cell-relative positions now and at the
-    // last rebuild, and the next step's rebuild word
-    const uint32_t* __restrict__ vl;
-    const uint32_t* __restrict__ vlc;
-    const float4* __restrict__ spos;
-    const float4* __restrict__ spb;
-    uint32_t* __restrict__ req_next;
-    unsigned long long* __restrict__ pair_counters;  // in-range unique pairs per workgroup slot
-    uint32_t* __restrict__ flags;
-    double dt, lim2;                   // time step; squared displacement that triggers a rebuild
+    // Verlet-list steps (SKIN): their buffers, read from device memory where they are used
+    // (as kernel arguments they would stay in scalar registers through the whole kernel)
+    const SkinArgs* __restrict__ sk;
+    int par;                           // step parity: the next step's rebuild word is sk->req[par ^ 1]
     Grid g;
     int n, chunk, nblocks, hyb_axis;
     int h16;                           // 1e-2 <= rc <= 1e4: the FP16 rotation filter is exact enough
@@ -452,6 +449,23 @@ __device__ __forceinline__ bool skin_exceeded(const double* __restrict__ vx, con
 constexpr int kAtmWaves = 4;
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
+}  // namespace
+
+// Buffers of the Verlet-list force pass (one copy per context in device memory).
+struct SkinArgs {
+    const uint32_t* vl;        // [N][kSkinCap] entries: cell-sorted slot | offset class << 25
+    const uint32_t* vlc;       // [N] lengths
+    const float4* spos;        // [N] FP32 positions relative to the build-time cell centres
+    const float4* spb;         // [N] the same at the last rebuild
+    const float4* skin_off;    // [125] offset of each offset class, image code in .w
+    uint32_t* req;             // [2] rebuild words of the two step parities
+    unsigned long long* pair_counters;  // in-range unique pairs per workgroup slot
+    uint32_t* flags;
+    double dt, lim2;           // time step; squared displacement that triggers a rebuild
+};
+
+namespace {
+
 // ATM: 1 physical, 2 reference semantics.  SKIN (physical, force pass, no hybrid): the
 // site's in-range neighbours come from its Verlet list and the current positions; the
 // kernel then writes the complete force (it is the step's only force kernel) and raises
@@ -475,16 +489,8 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
     const int lb = (b & 7) * a.chunk + (b >> 3);  // XCD-aware: contiguous site slabs per XCD
     const int p = lb * kAtmWaves + wvu;
     if (SKIN) {
-        // the 125 offset classes of the Verlet-list entries (the list pass's FP32 offsets)
-        const int t = threadIdx.x;
-        if (t < 125) {
-            const int cx = t % 5, cy = (t / 5) % 5, cz = t / 25;
-            auto img = [](int c) { return c == 3 ? 2u : (c == 4 ? 1u : 0u); };
-            woff[t] = make_float4(axis_offset(cx, a.g.nx, a.g.cs, a.g.size[0]),
-                                  axis_offset(cy, a.g.ny, a.g.cs, a.g.size[1]),
-                                  axis_offset(cz, a.g.nz, a.g.cs, a.g.size[2]),
-                                  __uint_as_float((img(cx) | (img(cy) << 2) | (img(cz) << 4)) << kIdxBits));
-        }
+        // the offsets of the 125 classes of the Verlet-list entries (k_skin_offsets)
+        if (threadIdx.x < 125) woff[threadIdx.x] = a.sk->skin_off[threadIdx.x];
         __syncthreads();
     }
     double e_acc = 0.0;
@@ -503,9 +509,11 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
             // so a fresh list gives the same vectors); within a few ulps of rc the FP64
             // distance decides, as in the list pass.  Both row halves are loaded at once
             // (the row is allocated to its capacity); slots past the length read site p.
-            const float4 qi = a.spos[p];
-            const int nv = __builtin_amdgcn_readfirstlane((int)a.vlc[p]);
-            const uint32_t* vrow = a.vl + (size_t)p * kSkinCap;
+            const SkinArgs* sk = a.sk;
+            const float4* spos = sk->spos;
+            const float4 qi = spos[p];
+            const int nv = __builtin_amdgcn_readfirstlane((int)sk->vlc[p]);
+            const uint32_t* vrow = sk->vl + (size_t)p * kSkinCap;
             const uint32_t e0 = vrow[lane], e1 = vrow[64 + lane];
             int cnt = 0;
 #pragma unroll
@@ -513,7 +521,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
                 const int s = h * 64 + lane;
                 const bool live = s < nv;
                 const uint32_t e = live ? (h ? e1 : e0) : (uint32_t)p;
-                const float4 q = a.spos[e & kSkinIdxMask];
+                const float4 q = spos[e & kSkinIdxMask];
                 const float4 o = woff[e >> kSkinIdxBits];
                 float4 d;
                 d.x = (q.x + o.x) - qi.x;
@@ -534,7 +542,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
                 }
                 cnt += (int)__popcll(bal);
             }
-            if (cnt > kFullCap && lane == 0) atomicOr(a.flags, FLAG_NL_OVERFLOW);
+            if (cnt > kFullCap && lane == 0) atomicOr(sk->flags, FLAG_NL_OVERFLOW);
             nn = min(cnt, kFullCap);
             wave_sync();
         } else {
@@ -730,9 +738,12 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
                 a.fy[oi] = nfy;
                 a.fz[oi] = nfz;
                 if (a.kick > 0.0) kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
-                if (SKIN && skin_exceeded(a.vx, a.vy, a.vz, a.mass, oi, a.kick, a.dt, nfx, nfy, nfz, a.spos[p],
-                                          a.spb[p], a.lim2))
-                    atomicOr(a.req_next, 1u);
+                if (SKIN) {
+                    const SkinArgs* sk = a.sk;
+                    if (skin_exceeded(a.vx, a.vy, a.vz, a.mass, oi, a.kick, sk->dt, nfx, nfy, nfz, sk->spos[p],
+                                      sk->spb[p], sk->lim2))
+                        atomicOr(sk->req + (a.par ^ 1), 1u);
+                }
                 if (a.esite) a.esite[oi] = HYB ? de * (double)li : de;
             }
             e_acc = HYB ? de * (double)li : de;
@@ -752,7 +763,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
         if (SKIN) {
             unsigned long long tp = 0;
             for (int k = 0; k < kAtmWaves; k++) tp += red_np[k];
-            atomicAdd(&a.pair_counters[b], tp);
+            atomicAdd(&a.sk->pair_counters[b], tp);
         }
         if (ENERGY) {
             double s = 0;
@@ -761,6 +772,18 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
         }
     }
     if (ENERGY) block_store_sym(a.vpart, a.nblocks, b, vir);
+}
+
+// The FP32 offset and image code of each of the 125 Verlet-list offset classes: the list
+// pass's nb_cell arithmetic per axis, computed once per run.
+__global__ void k_skin_offsets(Grid g, float4* __restrict__ out) {
+    const int t = threadIdx.x;
+    if (t >= 125) return;
+    const int cx = t % 5, cy = (t / 5) % 5, cz = t / 25;
+    auto img = [](int c) { return c == 3 ? 2u : (c == 4 ? 1u : 0u); };
+    out[t] = make_float4(axis_offset(cx, g.nx, g.cs, g.size[0]), axis_offset(cy, g.ny, g.cs, g.size[1]),
+                         axis_offset(cz, g.nz, g.cs, g.size[2]),
+                         __uint_as_float((img(cx) | (img(cy) << 2) | (img(cz) << 4)) << kIdxBits));
 }
 
 template <bool U, bool M, bool LJ, int NL, bool E, bool LIM>
@@ -859,6 +882,31 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, do
     return dispatch<false, true>(c, a, lj, nl, energy, lim);
 }
 
+hipError_t launch_skin_offsets(komd_ctx* c) {
+    k_skin_offsets<<<1, 128, 0, c->stream>>>(c->grid, c->d.skin_off);
+    return hipGetLastError();
+}
+
+size_t skin_args_bytes() { return sizeof(SkinArgs); }
+
+// The Verlet-list force pass's buffer table and constants, written before a skinned run.
+hipError_t upload_skin_args(komd_ctx* c) {
+    auto& d = c->d;
+    if (!d.vl || !d.vlc || !d.spb || !d.skin_off || !d.skin_req || !d.skin_args || c->n > (int64_t)kSkinIdxMask)
+        return hipErrorInvalidValue;
+    SkinArgs h{};
+    h.vl = d.vl; h.vlc = d.vlc; h.spos = d.spos; h.spb = d.spb; h.skin_off = d.skin_off;
+    h.req = d.skin_req;
+    h.pair_counters = d.cnt_trip + atm_blocks(c);
+    h.flags = d.flags;
+    h.dt = c->prm.delta_t;
+    // rebuild once a site would move further than half the skin, less a margin for the
+    // FP32 distance tests of the list pass
+    const double lim = 0.5 * c->skin - 1e-5 * (c->prm.cutoff + c->skin);
+    h.lim2 = lim > 0 ? lim * lim : 0.0;
+    return h2d(c, d.skin_args, &h, sizeof(h));
+}
+
 // The Verlet-list pass of a skinned step (parity par): every site within cutoff + skin,
 // run only when the step's rebuild word is set.
 hipError_t launch_skin_list(komd_ctx* c, int par) {
@@ -908,21 +956,14 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
     a.nu = (float)c->prm.nu;
     a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
     a.kick = (!energy && c->kick_in == KICK_IN_ATM) ? 0.5 * c->prm.delta_t : 0.0;
-    a.flags = d.flags;
     ScopedTimer tm(c, "atm");
     const bool hyb = lam != nullptr;
     if (skin_par >= 0) {
-        // Verlet-list step: physical ATM with LJ, NVE (fused half-kick), no hybrid
-        if (atm != 1 || energy || hyb || !a.lj || !(a.kick > 0.0) || !d.vl) return hipErrorInvalidValue;
-        if (!d.spb || c->n > (int64_t)kSkinIdxMask) return hipErrorInvalidValue;
-        a.vl = d.vl; a.vlc = d.vlc; a.spos = d.spos; a.spb = d.spb;
-        a.req_next = d.skin_req + (skin_par ^ 1);
-        a.pair_counters = d.cnt_trip + a.nblocks;
-        a.dt = c->prm.delta_t;
-        // rebuild once a site would move further than half the skin, less a margin for the
-        // FP32 distance tests of the list pass
-        const double lim = 0.5 * c->skin - 1e-5 * (c->prm.cutoff + c->skin);
-        a.lim2 = lim > 0 ? lim * lim : 0.0;
+        // Verlet-list step: physical ATM with LJ, NVE (fused half-kick), no hybrid; the
+        // buffers in d.skin_args (upload_skin_args)
+        if (atm != 1 || energy || hyb || !a.lj || !(a.kick > 0.0) || !d.skin_args) return hipErrorInvalidValue;
+        a.sk = d.skin_args;
+        a.par = skin_par;
         k_atm<1, false, false, true><<<a.nblocks, 64 * kAtmWaves, 0, c->stream>>>(a);
     } else if (hyb) {
         if (energy) k_atm<1, true, true><<<a.nblocks, 64 * kAtmWaves, 0, c->stream>>>(a);

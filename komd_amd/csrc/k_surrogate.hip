@@ -1717,8 +1717,8 @@ hipError_t ensure_train_buffers(komd_ctx* c) {
     if (!s->mom) {
         KOMD_CHECK_HIP(salloc(&s->mom, NPARAM));
         KOMD_CHECK_HIP(salloc(&s->vel, NPARAM));
-        KOMD_CHECK_HIP(hipMemset(s->mom, 0, NPARAM * 4));
-        KOMD_CHECK_HIP(hipMemset(s->vel, 0, NPARAM * 4));
+        KOMD_CHECK_HIP(dzero(c, s->mom, NPARAM * 4));
+        KOMD_CHECK_HIP(dzero(c, s->vel, NPARAM * 4));
         s->step = 0;
     }
     return hipSuccess;
@@ -1739,7 +1739,7 @@ komd_status komd_surrogate_set_params(komd_ctx* c, const float* params, int64_t 
         if (!std::isfinite((double)params[i])) return api_fail(c, KOMD_ERR_ARG, "non-finite surrogate parameter");
     Surrogate* s = sur_of(c);
     SCALL(c, ensure_site_buffers(c), "komd_surrogate_set_params");
-    SCALL(c, hipMemcpy(s->params, params, NPARAM * 4, hipMemcpyHostToDevice), "komd_surrogate_set_params");
+    SCALL(c, h2d(c, s->params, params, NPARAM * 4), "komd_surrogate_set_params");
     k_sur_pack<<<1, 256, 0, c->stream>>>(s->params, s->ops);
     SCALL(c, hipGetLastError(), "komd_surrogate_set_params");
     s->have_params = true;
@@ -1873,7 +1873,7 @@ komd_status komd_train_standardize(komd_ctx* c) {
     if (s->snap_r.empty()) return api_fail(c, KOMD_ERR_STATE, "no training snapshots");
     SCALL(c, ensure_site_buffers(c), "komd_train_standardize");
     if (!s->have_params) {
-        SCALL(c, hipMemset(s->params, 0, NPARAM * 4), "komd_train_standardize");
+        SCALL(c, dzero(c, s->params, NPARAM * 4), "komd_train_standardize");
         s->have_params = true;
     }
     SCALL(c, salloc(&s->stats, (size_t)kJudgeBlocks * 2 * ND), "komd_train_standardize");
@@ -1904,7 +1904,7 @@ komd_status komd_train_standardize(komd_ctx* c) {
         prm[O_SHIFT + j] = (float)mean;
         prm[O_SCALE + j] = (float)(1.0 / (sqrt(var) + 1e-6 * (fabs(mean) + 1e-12)));
     }
-    SCALL(c, hipMemcpy(s->params, prm.data(), NPARAM * 4, hipMemcpyHostToDevice), "komd_train_standardize");
+    SCALL(c, h2d(c, s->params, prm.data(), NPARAM * 4), "komd_train_standardize");
     k_sur_pack<<<1, 256, 0, c->stream>>>(s->params, s->ops);
     SCALL(c, hipGetLastError(), "komd_train_standardize");
     return api_sync(c, "komd_train_standardize");

@@ -237,9 +237,9 @@ hipError_t table_upload(komd_ctx* c, int64_t nodes, const double* x, const doubl
     hipError_t e = hipMalloc((void**)&t->tx, tx.size() * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&t->seg, seg.size() * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc((void**)&t->useg, useg.size() * sizeof(float2));
-    if (e == hipSuccess) e = hipMemcpy(t->tx, tx.data(), tx.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(t->seg, seg.data(), seg.size() * sizeof(float4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(t->useg, useg.data(), useg.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = h2d(c, t->tx, tx.data(), tx.size() * sizeof(float));
+    if (e == hipSuccess) e = h2d(c, t->seg, seg.data(), seg.size() * sizeof(float4));
+    if (e == hipSuccess) e = h2d(c, t->useg, useg.data(), useg.size() * sizeof(float2));
     if (e != hipSuccess) {
         PairTable* old = c->tab;
         c->tab = t;

@@ -67,7 +67,7 @@ void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
                     d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.scanp, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
-                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf, d.vl, d.vlc, d.spb, d.skin_req};
+                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf, d.vl, d.vlc, d.spb, d.skin_off, d.skin_args, d.skin_req};
     for (void* p : ptrs)
         if (p) hipFree(p);
     DevState keep;
@@ -115,8 +115,7 @@ komd_status sync_check(komd_ctx* c, const char* where) {
     e = hipMemcpy(&flags, c->d.flags, sizeof(uint32_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(c, e, where);
     if (flags) {
-        uint32_t zero = 0;
-        hipMemcpy(c->d.flags, &zero, sizeof(uint32_t), hipMemcpyHostToDevice);
+        dzero(c, c->d.flags, sizeof(uint32_t));
         if (flags & FLAG_NL_OVERFLOW)
             return fail(c, KOMD_ERR_OVERFLOW,
                         std::string(where) + ": a site has more than " + std::to_string(kFullCap) +
@@ -153,8 +152,8 @@ komd_status read_counters(komd_ctx* c, int64_t out[2], bool reset) {
     out[0] = (int64_t)pairs;
     out[1] = (int64_t)trip;
     if (reset) {
-        HIPCALL(c, hipMemset(c->d.cnt_pair, 0, hp.size() * 8), "komd_counters");
-        HIPCALL(c, hipMemset(c->d.cnt_trip, 0, ht.size() * 8), "komd_counters");
+        HIPCALL(c, dzero(c, c->d.cnt_pair, hp.size() * 8), "komd_counters");
+        HIPCALL(c, dzero(c, c->d.cnt_trip, ht.size() * 8), "komd_counters");
     }
     return KOMD_OK;
 }
@@ -183,9 +182,11 @@ hipError_t ensure_skin(komd_ctx* c) {
     if (!d.vl) KOMD_CHECK_HIP(dalloc(&d.vl, (size_t)c->n * kSkinCap));
     if (!d.vlc) KOMD_CHECK_HIP(dalloc(&d.vlc, (size_t)c->n));
     if (!d.spb) KOMD_CHECK_HIP(dalloc(&d.spb, (size_t)c->n));
+    if (!d.skin_off) KOMD_CHECK_HIP(dalloc(&d.skin_off, 125));
+    if (!d.skin_args) KOMD_CHECK_HIP(hipMalloc((void**)&d.skin_args, skin_args_bytes()));
     if (!d.skin_req) {
         KOMD_CHECK_HIP(dalloc(&d.skin_req, 4));
-        KOMD_CHECK_HIP(hipMemset(d.skin_req, 0, 4 * sizeof(uint32_t)));
+        KOMD_CHECK_HIP(dzero(c, d.skin_req, 4 * sizeof(uint32_t)));
     }
     return hipSuccess;
 }
@@ -374,9 +375,9 @@ komd_status komd_create(const komd_box* box, const komd_params* params, komd_ctx
     if (e == hipSuccess) e = dalloc(&c->d.scal, 32);
     if (e == hipSuccess) e = dalloc(&c->d.flags, 1);
     if (e == hipSuccess) e = dalloc(&c->d.counters, 2);
-    if (e == hipSuccess) e = hipMemset(c->d.flags, 0, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(c->d.counters, 0, 2 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->d.scal, 0, 32 * sizeof(double));
+    if (e == hipSuccess) e = dzero(c, c->d.flags, sizeof(uint32_t));
+    if (e == hipSuccess) e = dzero(c, c->d.counters, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = dzero(c, c->d.scal, 32 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->flag_host, 2 * sizeof(uint32_t));
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&c->flag_ev[k], hipEventDisableTiming);
     if (e == hipSuccess) {
@@ -503,28 +504,28 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     if (e == hipSuccess) e = dalloc(&d.cnt_pair, (size_t)cell_blocks(c));
     // triplet visits, then the in-range pairs of Verlet-list ATM passes
     if (e == hipSuccess) e = dalloc(&d.cnt_trip, (size_t)2 * atm_blocks(c));
-    if (e == hipSuccess) e = hipMemset(d.cnt_pair, 0, (size_t)cell_blocks(c) * 8);
-    if (e == hipSuccess) e = hipMemset(d.cnt_trip, 0, (size_t)2 * atm_blocks(c) * 8);
+    if (e == hipSuccess) e = dzero(c, d.cnt_pair, (size_t)cell_blocks(c) * 8);
+    if (e == hipSuccess) e = dzero(c, d.cnt_trip, (size_t)2 * atm_blocks(c) * 8);
     if (e != hipSuccess) {
         free_sites(c);
         return fail(c, KOMD_ERR_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
     }
     for (int k = 0; k < 3 && e == hipSuccess; k++) {
         for (int64_t i = 0; i < n; i++) buf[(size_t)i] = r[3 * i + k];
-        e = hipMemcpy(k == 0 ? d.rx : (k == 1 ? d.ry : d.rz), buf.data(), nb, hipMemcpyHostToDevice);
+        e = h2d(c, k == 0 ? d.rx : (k == 1 ? d.ry : d.rz), buf.data(), nb);
         for (int64_t i = 0; i < n && e == hipSuccess; i++) buf[(size_t)i] = v[3 * i + k];
         if (e == hipSuccess)
-            e = hipMemcpy(k == 0 ? d.vx : (k == 1 ? d.vy : d.vz), buf.data(), nb, hipMemcpyHostToDevice);
+            e = h2d(c, k == 0 ? d.vx : (k == 1 ? d.vy : d.vz), buf.data(), nb);
     }
-    if (e == hipSuccess) e = hipMemcpy(d.mass, mass, nb, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.eps, eps, nb, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.sig, sig, nb, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.idcls, idc.data(), (size_t)n * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.mol_start, mstart.data(), mstart.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(d.fx, 0, nb);
-    if (e == hipSuccess) e = hipMemset(d.fy, 0, nb);
-    if (e == hipSuccess) e = hipMemset(d.fz, 0, nb);
-    if (e == hipSuccess) e = hipMemset(d.count, 0, C * 4);
+    if (e == hipSuccess) e = h2d(c, d.mass, mass, nb);
+    if (e == hipSuccess) e = h2d(c, d.eps, eps, nb);
+    if (e == hipSuccess) e = h2d(c, d.sig, sig, nb);
+    if (e == hipSuccess) e = h2d(c, d.idcls, idc.data(), (size_t)n * 4);
+    if (e == hipSuccess) e = h2d(c, d.mol_start, mstart.data(), mstart.size() * 4);
+    if (e == hipSuccess) e = dzero(c, d.fx, nb);
+    if (e == hipSuccess) e = dzero(c, d.fy, nb);
+    if (e == hipSuccess) e = dzero(c, d.fz, nb);
+    if (e == hipSuccess) e = dzero(c, d.count, C * 4);
     if (e != hipSuccess) {
         free_sites(c);
         return hip_fail(c, e, "komd_set_sites");
@@ -545,7 +546,7 @@ komd_status komd_set_state(komd_ctx* c, const double* r, const double* v, const 
         if (!src[q]) continue;
         for (int k = 0; k < 3; k++) {
             for (int64_t i = 0; i < n; i++) buf[(size_t)i] = src[q][3 * i + k];
-            HIPCALL(c, hipMemcpy(dst[q][k], buf.data(), (size_t)n * 8, hipMemcpyHostToDevice), "komd_set_state");
+            HIPCALL(c, h2d(c, dst[q][k], buf.data(), (size_t)n * 8), "komd_set_state");
         }
     }
     c->nl_valid = false;
@@ -618,14 +619,14 @@ komd_status komd_set_topology(komd_ctx* c, int64_t nb, const uint32_t* bonds, do
         }
         HIPCALL(c, dalloc(&c->d.bond_mol, b_start.size()), "komd_set_topology");
         HIPCALL(c, dalloc(&c->d.angle_mol, a_start.size()), "komd_set_topology");
-        HIPCALL(c, hipMemcpy(c->d.bond_mol, b_start.data(), b_start.size() * 4, hipMemcpyHostToDevice), "komd_set_topology");
-        HIPCALL(c, hipMemcpy(c->d.angle_mol, a_start.data(), a_start.size() * 4, hipMemcpyHostToDevice), "komd_set_topology");
+        HIPCALL(c, h2d(c, c->d.bond_mol, b_start.data(), b_start.size() * 4), "komd_set_topology");
+        HIPCALL(c, h2d(c, c->d.angle_mol, a_start.data(), a_start.size() * 4), "komd_set_topology");
     }
     c->bonded_by_mol = by_mol;
     HIPCALL(c, dalloc(&c->d.bonds, (size_t)std::max<int64_t>(1, 2 * nb)), "komd_set_topology");
     HIPCALL(c, dalloc(&c->d.angles, (size_t)std::max<int64_t>(1, 3 * na)), "komd_set_topology");
-    if (nb) HIPCALL(c, hipMemcpy(c->d.bonds, b_sorted.data(), (size_t)(2 * nb) * 4, hipMemcpyHostToDevice), "komd_set_topology");
-    if (na) HIPCALL(c, hipMemcpy(c->d.angles, a_sorted.data(), (size_t)(3 * na) * 4, hipMemcpyHostToDevice), "komd_set_topology");
+    if (nb) HIPCALL(c, h2d(c, c->d.bonds, b_sorted.data(), (size_t)(2 * nb) * 4), "komd_set_topology");
+    if (na) HIPCALL(c, h2d(c, c->d.angles, a_sorted.data(), (size_t)(3 * na) * 4), "komd_set_topology");
     c->nbonds = nb;
     c->nangles = na;
     c->kb = kb;
@@ -824,6 +825,8 @@ struct FlagWatch {
 // positions are wrapped into the box and the grid matches them, as after a plain run.
 komd_status run_skin(komd_ctx* c, int64_t nsteps) {
     HIPCALL(c, ensure_skin(c), "komd_run");
+    HIPCALL(c, launch_skin_offsets(c), "komd_run");
+    HIPCALL(c, upload_skin_args(c), "komd_run");
     HIPCALL(c, launch_skin_arm(c, c->skin_par), "komd_run");
     if (c->timing || nsteps < 3) {
         for (int64_t k = 0; k < nsteps; k++) {
@@ -977,7 +980,7 @@ komd_status komd_set_skin(komd_ctx* c, double skin) {
                         std::to_string(c->prm.cutoff + skin) + ")");
     c->skin = skin;
     drop_graph(c);
-    if (c->d.skin_req) HIPCALL(c, hipMemset(c->d.skin_req + 2, 0, sizeof(uint32_t)), "komd_set_skin");
+    if (c->d.skin_req) HIPCALL(c, dzero(c, c->d.skin_req + 2, sizeof(uint32_t)), "komd_set_skin");
     return KOMD_OK;
 }
 
@@ -1066,8 +1069,8 @@ komd_status measure_observables(komd_ctx* c, int32_t kind3b, komd_observables* o
         if ((s = add_virial(atm_blocks(c)))) return s;
     }
     if (c->nbonds + c->nangles > 0) {
-        HIPCALL(c, hipMemset(d.red, 0, 512 * sizeof(double)), "komd_measure_observables");
-        HIPCALL(c, hipMemset(d.vpart, 0, (size_t)kSym * 512 * sizeof(double)), "komd_measure_observables");
+        HIPCALL(c, dzero(c, d.red, 512 * sizeof(double)), "komd_measure_observables");
+        HIPCALL(c, dzero(c, d.vpart, (size_t)kSym * 512 * sizeof(double)), "komd_measure_observables");
         HIPCALL(c, launch_bonded(c, true), "komd_measure_observables");
         if ((s = reduce(d.red, 256, 2))) return s;
         out->u_bond = vals[0];

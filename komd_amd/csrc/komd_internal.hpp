@@ -61,6 +61,7 @@ struct Partials {
 };
 
 struct Surrogate;  // k_surrogate.hip
+struct SkinArgs;   // k_cell.hip
 struct PairTable;  // k_table.hip
 
 struct DevState {
@@ -86,6 +87,8 @@ struct DevState {
     uint32_t* vl = nullptr;       // [N][kSkinCap]
     uint32_t* vlc = nullptr;      // [N] lengths
     float4* spb = nullptr;        // [N] cell-sorted FP32 positions of the last rebuild
+    float4* skin_off = nullptr;   // [125] FP32 offset (and image code) of each list offset class
+    SkinArgs* skin_args = nullptr;  // the Verlet-list force pass's buffer table (device copy)
     uint32_t* skin_req = nullptr; // [0], [1] rebuild words, one per step parity; [2] rebuilds so far
     // bonded
     uint32_t *bonds = nullptr, *angles = nullptr;
@@ -192,6 +195,16 @@ struct ScopedTimer {
     }
 };
 
+// ---- host <-> device transfers on the context stream ---------------------
+// The context stream is non-blocking, so nothing orders null-stream memsets and copies
+// with the kernels queued on it: every transfer into device memory goes through the
+// context stream.  h2d returns once the copy is complete (the source may be reused).
+inline hipError_t h2d(komd_ctx* c, void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+inline hipError_t dzero(komd_ctx* c, void* dst, size_t bytes) { return hipMemsetAsync(dst, 0, bytes, c->stream); }
+
 // ---- launch wrappers --------------------------------------------------
 // k_container.hip
 hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift, bool reset_f = true);
@@ -247,6 +260,9 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, do
 hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite,
                       bool lj = false, int skin_par = -1);
 hipError_t launch_skin_list(komd_ctx* c, int par);
+hipError_t launch_skin_offsets(komd_ctx* c);  // d.skin_off from the grid
+hipError_t upload_skin_args(komd_ctx* c);     // d.skin_args (before a skinned run)
+size_t skin_args_bytes();
 int cell_blocks(komd_ctx* c);
 int atm_blocks(komd_ctx* c);
 // k_table.hip: tabulated pair force over current neighbour lists; with energy the pair

@@ -214,14 +214,15 @@ def config_chains_262144(seed: int = 1002) -> System:
     return chains((64, 32, 32), 4, 0.7, seed=seed)
 
 
-def config_atm_32000(seed: int = 1003) -> System:
-    """32,000 argon atoms, FCC 20^3 x 4, rho 0.8, T* 0.8 (LJ + ATM)."""
-    return fcc(20, 0.8, 0.8, seed)
+def config_atm_32000(seed: int = 1003, list_radius: float = 2.5) -> System:
+    """32,000 argon atoms, FCC 20^3 x 4, rho 0.8, T* 0.8 (LJ + ATM); cells of edge at
+    least list_radius (cutoff 2.5 plus the Verlet-list skin, if any)."""
+    return fcc(20, 0.8, 0.8, seed, rc=list_radius)
 
 
-def config_atm_256000(seed: int = 1004) -> System:
+def config_atm_256000(seed: int = 1004, list_radius: float = 2.5) -> System:
     """256,000 argon atoms, FCC 40^3 x 4, rho 0.8, T* 0.8."""
-    return fcc(40, 0.8, 0.8, seed)
+    return fcc(40, 0.8, 0.8, seed, rc=list_radius)
 
 
 def config_slab_1048576(seed: int = 1005) -> System:
