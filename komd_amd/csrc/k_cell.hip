@@ -66,6 +66,7 @@ struct CellArgs {
     uint16_t* __restrict__ nld;    // [N][kFullCap] neighbour cell direction | rank in that cell << 5
     uint32_t* __restrict__ vl;     // NL 3: [N][kSkinCap] Verlet list (cell-sorted slot | direction << 26)
     uint32_t* __restrict__ vlc;    // NL 3: [N] its length
+    uint16_t* __restrict__ vld;    // NL 4: [N][kSkinCap] neighbour cell direction | rank in that cell << 5
     const uint32_t* __restrict__ req;  // NL 3: rebuild word of the step (0: the lists stay)
     double* __restrict__ part;
     double* __restrict__ vpart;    // [kSym][nblocks] virial tensor partials (energy mode)
@@ -161,15 +162,17 @@ __device__ __forceinline__ float4 staged(float4 v, const float* off, uint32_t co
 }
 
 // NL: 0 none, 1 neighbour list, 2 neighbour list + cell directions, 3 Verlet list
-// (every site within cutoff + skin; the step's rebuild word gates the launch).
+// (every site within cutoff + skin; the step's rebuild word gates the launch), 4 Verlet
+// list + cell directions.
 template <bool UNIFORM, bool MOLS, bool LJ, int NL, bool ENERGY, bool LIMIT>
 __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
-    if (NL == 3 && a.req[0] == 0u) return;
+    constexpr bool VL = NL >= 3;
+    if (VL && a.req[0] == 0u) return;
     constexpr int kStage = kChunkLJ;
     constexpr int kWaves = kCellThreads / 64;
     __shared__ float4 pos[kStage + 32];  // + 32: a group's lanes past the chunk read in bounds
-    __shared__ uint16_t sdir[NL == 2 ? kStage : 1];
-    __shared__ uint32_t sref[NL == 3 ? kStage : 1];  // cell-sorted slot | offset class << 25
+    __shared__ uint16_t sdir[(NL == 2 || NL == 4) ? kStage : 1];
+    __shared__ uint32_t sref[VL ? kStage : 1];  // cell-sorted slot | offset class << 25
     __shared__ float2 cprm[UNIFORM ? 1 : kStage];
     __shared__ uint32_t cmol[MOLS ? kStage : 1];
     __shared__ int nb_start[27];
@@ -257,8 +260,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 pos[t] = staged(a.spos[j], nb_off[k], nb_code[k]);
                 if (!UNIFORM) cprm[t] = a.sprm[j];
                 if (MOLS) cmol[t] = a.sid[j];
-                if (NL == 2) sdir[t] = (uint16_t)(k | ((gidx - nb_pref[k]) << 5));
-                if (NL == 3) sref[t] = (uint32_t)j | (nb_cls[k] << kSkinIdxBits);
+                if (NL == 2 || NL == 4) sdir[t] = (uint16_t)(k | ((gidx - nb_pref[k]) << 5));
+                if (VL) sref[t] = (uint32_t)j | (nb_cls[k] << kSkinIdxBits);
             }
             __syncthreads();
             // every lane of the workgroup runs the same trip count (ballots need the whole wave)
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 // compare, which the ballots take as a lane mask directly
                 const float r2 = cand ? dx * dx + dy * dy + dz * dz : __builtin_huge_valf();
                 bool in;
-                if (NL == 3) {
+                if (VL) {
                     // Verlet list: a superset bound, no exact decision needed
                     in = r2 <= a.rl2f;
                 } else {
@@ -284,13 +287,16 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 }
                 const uint32_t oh = in ? oj : oi;  // hit: in range and not i itself
                 const bool hit = oh != oi;
-                if (NL != 3) npairs += (unsigned long long)__popcll(ballot(oh > oi));  // wave-uniform
-                if (NL == 3) {
+                if (!VL) npairs += (unsigned long long)__popcll(ballot(oh > oi));  // wave-uniform
+                if (VL) {
                     const unsigned long long bal = ballot(hit) & gmask;
                     if (hit) {
                         const uint32_t s = ncnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                        if (s < (uint32_t)kSkinCap) a.vl[(size_t)p * kSkinCap + s] = sref[t];
+                        if (s < (uint32_t)kSkinCap) {
+                            a.vl[(size_t)p * kSkinCap + s] = sref[t];
+                            if (NL == 4) a.vld[(size_t)p * kSkinCap + s] = sdir[t];
+                        }
                     }
                     ncnt += (uint32_t)__popcll(bal);
                 } else if (NL) {
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                 }
             }
         }
-        if (NL == 3 && active && sub == 0) {
+        if (VL && active && sub == 0) {
             a.vlc[p] = min(ncnt, (uint32_t)kSkinCap);
             if (ncnt > (uint32_t)kSkinCap) atomicOr(a.flags, FLAG_VL_OVERFLOW);
         } else if (NL && active && sub == 0) {
@@ -393,9 +399,6 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-}  // namespace
-struct SkinArgs;
-namespace {
 struct AtmArgs {
     const float4* __restrict__ nl;
     const uint32_t* __restrict__ nlc;
@@ -430,41 +433,8 @@ struct AtmArgs {
     double rc2, kick;
 };
 
-// Would the next Integrator::Step0 (Integrator.cpp:28-32) move site i more than sqrt(lim2)
-// from its build position?  The step (v + f dt/(2m)) dt of the drift, after this step's
-// second half-kick has been written to v, plus the displacement since the rebuild from the
-// FP32 cell-relative positions (both relative to the same build-time cell centre; their
-// rounding, below 3e-7, lies inside the margin of lim2).
-__device__ __forceinline__ bool skin_exceeded(const double* __restrict__ vx, const double* __restrict__ vy,
-                                              const double* __restrict__ vz, const double* __restrict__ mass,
-                                              uint32_t i, double dt_half, double dt, double fx, double fy, double fz,
-                                              float4 q, float4 qb, double lim2) {
-    const double k = dt_half / mass[i];
-    const double x = (double)(q.x - qb.x) + (vx[i] + fx * k) * dt;
-    const double y = (double)(q.y - qb.y) + (vy[i] + fy * k) * dt;
-    const double z = (double)(q.z - qb.z) + (vz[i] + fz * k) * dt;
-    return !(x * x + y * y + z * z <= lim2);
-}
-
 constexpr int kAtmWaves = 4;
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-
-}  // namespace
-
-// Buffers of the Verlet-list force pass (one copy per context in device memory).
-struct SkinArgs {
-    const uint32_t* vl;        // [N][kSkinCap] entries: cell-sorted slot | offset class << 25
-    const uint32_t* vlc;       // [N] lengths
-    const float4* spos;        // [N] FP32 positions relative to the build-time cell centres
-    const float4* spb;         // [N] the same at the last rebuild
-    const float4* skin_off;    // [125] offset of each offset class, image code in .w
-    uint32_t* req;             // [2] rebuild words of the two step parities
-    unsigned long long* pair_counters;  // in-range unique pairs per workgroup slot
-    uint32_t* flags;
-    double dt, lim2;           // time step; squared displacement that triggers a rebuild
-};
-
-namespace {
 
 // ATM: 1 physical, 2 reference semantics.  SKIN (physical, force pass, no hybrid): the
 // site's in-range neighbours come from its Verlet list and the current positions; the
@@ -786,6 +756,109 @@ __global__ void k_skin_offsets(Grid g, float4* __restrict__ out) {
                          __uint_as_float((img(cx) | (img(cy) << 2) | (img(cz) << 4)) << kIdxBits));
 }
 
+// ---------------------------------------------------------------------------
+// Verlet-list step of the learned and hybrid three-body kinds: each site's in-range
+// neighbours at the current positions, filtered from its Verlet list as the Verlet-list ATM
+// pass does (the same FP32 vectors and the FP64 decision near the cutoff), compacted into
+// the neighbour-list rows the three-body kernels read (r_j - r_i with the site word, the cell
+// direction and rank), and the LJ 12-6 force of those pairs written as the site's force:
+// the first force term of the step (uniform sites, no clamp).  One wave per site.
+struct RefreshArgs {
+    const SkinArgs* __restrict__ sk;
+    const uint32_t* __restrict__ sorted;
+    const double* __restrict__ rx;
+    const double* __restrict__ ry;
+    const double* __restrict__ rz;
+    float4* __restrict__ nl;
+    uint32_t* __restrict__ nlc;
+    uint16_t* __restrict__ nld;
+    double* __restrict__ fx;
+    double* __restrict__ fy;
+    double* __restrict__ fz;
+    Grid g;
+    int n, chunk;
+    float rc2f, band, sig2, eps24;
+    double rc2;
+};
+
+__global__ __launch_bounds__(64 * kAtmWaves) void k_skin_refresh(RefreshArgs a) {
+    __shared__ float4 woff[125];
+    __shared__ unsigned long long red_np[kAtmWaves];
+    const int lane = threadIdx.x & 63;
+    const int wvu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const SkinArgs* sk = a.sk;
+    if (threadIdx.x < 125) woff[threadIdx.x] = sk->skin_off[threadIdx.x];
+    __syncthreads();
+    const int b = blockIdx.x;
+    const int p = ((b & 7) * a.chunk + (b >> 3)) * kAtmWaves + wvu;  // XCD-aware site slabs
+    unsigned long long npair = 0;
+    if (p < a.n) {
+        const uint32_t oi = a.sorted[p];
+        const float4* spos = sk->spos;
+        const float4 qi = spos[p];
+        const int nv = __builtin_amdgcn_readfirstlane((int)sk->vlc[p]);
+        const uint32_t* vrow = sk->vl + (size_t)p * kSkinCap;
+        const uint16_t* drow = sk->vld + (size_t)p * kSkinCap;
+        float4* out = a.nl + (size_t)p * kFullCap;
+        uint16_t* dout = a.nld + (size_t)p * kFullCap;
+        const uint32_t e0 = vrow[lane], e1 = vrow[64 + lane];
+        double ljx = 0.0, ljy = 0.0, ljz = 0.0;
+        int cnt = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int s = h * 64 + lane;
+            const bool live = s < nv;
+            const uint32_t e = live ? (h ? e1 : e0) : (uint32_t)p;
+            const float4 q = spos[e & kSkinIdxMask];
+            const float4 o = woff[e >> kSkinIdxBits];
+            float4 d;
+            d.x = (q.x + o.x) - qi.x;
+            d.y = (q.y + o.y) - qi.y;
+            d.z = (q.z + o.z) - qi.z;
+            const uint32_t wj = __float_as_uint(q.w) | __float_as_uint(o.w);
+            d.w = __uint_as_float(wj);
+            const float r2 = live ? d.x * d.x + d.y * d.y + d.z * d.z : __builtin_huge_valf();
+            bool in = r2 <= a.rc2f;
+            const bool amb = fabsf(r2 - a.rc2f) <= a.band;
+            if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, oi, wj) > a.rc2);
+            const unsigned long long bal = ballot(in);
+            npair += (unsigned long long)__popcll(ballot(in && (wj & kIdxMask) > oi));
+            if (in) {
+                const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (slot < kFullCap) {
+                    out[slot] = d;
+                    dout[slot] = drow[s];
+                }
+                // LJ 12-6 of the pair (the pair pass's expression): force on i = -x fac
+                const float iX = __builtin_amdgcn_rcpf(r2);
+                const float s2 = a.sig2 * iX;
+                const float s6 = s2 * s2 * s2;
+                const float fac = a.eps24 * (2.f * s6 * s6 - s6) * iX;
+                ljx -= (double)(d.x * fac);
+                ljy -= (double)(d.y * fac);
+                ljz -= (double)(d.z * fac);
+            }
+            cnt += (int)__popcll(bal);
+        }
+        const double fxs = wave_sum(ljx), fys = wave_sum(ljy), fzs = wave_sum(ljz);
+        if (lane == 0) {
+            a.nlc[p] = (uint32_t)min(cnt, kFullCap);
+            if (cnt > kFullCap) atomicOr(sk->flags, FLAG_NL_OVERFLOW);
+            a.fx[oi] = fxs;
+            a.fy[oi] = fys;
+            a.fz[oi] = fzs;
+        }
+    }
+    if (lane == 0) red_np[wvu] = npair;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tp = 0;
+        for (int k = 0; k < kAtmWaves; k++) tp += red_np[k];
+        atomicAdd(&sk->pair_counters[b], tp);
+    }
+}
+
 template <bool U, bool M, bool LJ, int NL, bool E, bool LIM>
 void go(komd_ctx* c, const CellArgs& a) {
     k_cell<U, M, LJ, NL, E, LIM><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
@@ -895,7 +968,7 @@ hipError_t upload_skin_args(komd_ctx* c) {
     if (!d.vl || !d.vlc || !d.spb || !d.skin_off || !d.skin_req || !d.skin_args || c->n > (int64_t)kSkinIdxMask)
         return hipErrorInvalidValue;
     SkinArgs h{};
-    h.vl = d.vl; h.vlc = d.vlc; h.spos = d.spos; h.spb = d.spb; h.skin_off = d.skin_off;
+    h.vl = d.vl; h.vlc = d.vlc; h.vld = d.vld; h.spos = d.spos; h.spb = d.spb; h.skin_off = d.skin_off;
     h.req = d.skin_req;
     h.pair_counters = d.cnt_trip + atm_blocks(c);
     h.flags = d.flags;
@@ -909,9 +982,10 @@ hipError_t upload_skin_args(komd_ctx* c) {
 
 // The Verlet-list pass of a skinned step (parity par): every site within cutoff + skin,
 // run only when the step's rebuild word is set.
-hipError_t launch_skin_list(komd_ctx* c, int par) {
+hipError_t launch_skin_list(komd_ctx* c, int par, bool dirs) {
     auto& d = c->d;
     CellArgs a{};
+    a.vld = d.vld;
     a.spos = d.spos; a.sprm = d.sprm; a.sid = d.sid; a.start = d.start;
     a.rx = d.rx; a.ry = d.ry; a.rz = d.rz; a.eps = d.eps; a.sig = d.sig;
     a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
@@ -929,7 +1003,34 @@ hipError_t launch_skin_list(komd_ctx* c, int par) {
     if (!d.vl || !d.vlc || !d.skin_req || !c->uniform || c->multisite || c->n > (int64_t)kSkinIdxMask)
         return hipErrorInvalidValue;
     ScopedTimer tm(c, "cell_nl");
-    k_cell<true, false, false, 3, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
+    if (dirs) {
+        if (!d.vld) return hipErrorInvalidValue;
+        k_cell<true, false, false, 4, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
+    } else {
+        k_cell<true, false, false, 3, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_skin_refresh(komd_ctx* c) {
+    auto& d = c->d;
+    if (!d.skin_args || !d.vld || !d.nl || !d.nlc || !d.nld || !c->uniform || c->multisite) return hipErrorInvalidValue;
+    RefreshArgs a{};
+    a.sk = d.skin_args;
+    a.sorted = d.sorted; a.rx = d.rx; a.ry = d.ry; a.rz = d.rz;
+    a.nl = d.nl; a.nlc = d.nlc; a.nld = d.nld;
+    a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
+    a.g = c->grid;
+    a.n = (int)c->n;
+    const int blocks = atm_blocks(c);
+    a.chunk = blocks / 8;
+    a.rc2 = c->prm.cutoff * c->prm.cutoff;
+    a.rc2f = (float)a.rc2;
+    a.band = 4e-6f * a.rc2f;
+    a.sig2 = c->sig_u * c->sig_u;
+    a.eps24 = 24.f * c->eps_u;
+    ScopedTimer tm(c, "skin_refresh");
+    k_skin_refresh<<<blocks, 64 * kAtmWaves, 0, c->stream>>>(a);
     return hipGetLastError();
 }
 

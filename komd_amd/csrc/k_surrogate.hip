@@ -643,6 +643,8 @@ struct ForceArgs {
     double* __restrict__ vy;
     double* __restrict__ vz;
     const double* __restrict__ mass;
+    const SkinArgs* __restrict__ sk;  // Verlet-list step: the next step's rebuild decision (or null)
+    int par;                          // its parity
     Grid g;
     int n, hyb_axis, chunk, vparts;
     float pirc, cF, lF;
@@ -916,7 +918,13 @@ __global__ __launch_bounds__(256) void k_sur_apply(ForceArgs a) {
             a.fx[oi] = nfx;
             a.fy[oi] = nfy;
             a.fz[oi] = nfz;
-            if (a.kick > 0.0) kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
+            if (a.kick > 0.0) {
+                kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
+                const SkinArgs* sk = a.sk;
+                if (sk && skin_exceeded(a.vx, a.vy, a.vz, a.mass, oi, a.kick, sk->dt, nfx, nfy, nfz, sk->spos[p],
+                                        sk->spb[p], sk->lim2))
+                    atomicOr(sk->req + (a.par ^ 1), 1u);
+            }
         } else if (MODE == FORCE_OUT) {
             a.fout[3 * (size_t)oi] = F.x;
             a.fout[3 * (size_t)oi + 1] = F.y;
@@ -1501,6 +1509,10 @@ hipError_t surrogate_pass(komd_ctx* c, bool energy, bool hybrid, int mode) {
     } else {
         a.vx = c->d.vx; a.vy = c->d.vy; a.vz = c->d.vz; a.mass = c->d.mass;
         a.kick = (!energy && c->kick_in == KICK_IN_APPLY) ? 0.5 * c->prm.delta_t : 0.0;
+        if (a.kick > 0.0 && c->skin_step_par >= 0) {  // Verlet-list step: decide the next rebuild
+            a.sk = c->d.skin_args;
+            a.par = c->skin_step_par;
+        }
         k_sur_apply<FORCE_APPLY><<<site_blocks(c), 256, 0, c->stream>>>(a);
     }
     return hipGetLastError();

@@ -67,7 +67,7 @@ void free_sites(komd_ctx* c) {
     auto& d = c->d;
     void* ptrs[] = {d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.eps, d.sig, d.idcls, d.mol_start,
                     d.key, d.slot, d.count, d.start, d.tmp, d.sorted, d.inv, d.scanp, d.spos, d.sprm, d.sid, d.nl, d.nlc, d.nld, d.rows, d.red, d.cnt_pair, d.cnt_trip,
-                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf, d.vl, d.vlc, d.spb, d.skin_off, d.skin_args, d.skin_req};
+                    d.esite, d.vpart, d.comx, d.comy, d.comz, d.rdf, d.vl, d.vlc, d.vld, d.spb, d.skin_off, d.skin_args, d.skin_req};
     for (void* p : ptrs)
         if (p) hipFree(p);
     DevState keep;
@@ -176,10 +176,12 @@ hipError_t ensure_nl(komd_ctx* c, bool dirs) {
     return hipSuccess;
 }
 
-// Verlet-list buffers, allocated on the first skinned run.
-hipError_t ensure_skin(komd_ctx* c) {
+// Verlet-list buffers, allocated on the first skinned run (dirs: the entries' cell
+// directions and ranks, for the learned and hybrid kinds).
+hipError_t ensure_skin(komd_ctx* c, bool dirs) {
     auto& d = c->d;
     if (!d.vl) KOMD_CHECK_HIP(dalloc(&d.vl, (size_t)c->n * kSkinCap));
+    if (dirs && !d.vld) KOMD_CHECK_HIP(dalloc(&d.vld, (size_t)c->n * kSkinCap));
     if (!d.vlc) KOMD_CHECK_HIP(dalloc(&d.vlc, (size_t)c->n));
     if (!d.spb) KOMD_CHECK_HIP(dalloc(&d.spb, (size_t)c->n));
     if (!d.skin_off) KOMD_CHECK_HIP(dalloc(&d.skin_off, 125));
@@ -282,24 +284,37 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     return hipSuccess;
 }
 
-// Runs that use Verlet lists (komd_set_skin): the exact ATM with LJ summed in the three-body
-// kernel (uniform single-site sites, LJ 12-6, no clamp), NVE, no bonded terms.
+// Runs that use Verlet lists (komd_set_skin): the exact ATM, the surrogate or the hybrid slab
+// with LJ 12-6 of uniform single-site sites without the clamp, NVE, no bonded terms.
 bool skin_run(const komd_ctx* c, int kind3b, bool bonded, bool thermo) {
-    return c->skin > 0 && c->n <= (int64_t)kSkinIdxMask && kind3b == KOMD_3B_ATM_EXACT && !bonded && !thermo && c->pair_kind == KOMD_PAIR_LJ12_6 &&
-           c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+    const bool kind = kind3b == KOMD_3B_ATM_EXACT || kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID;
+    return c->skin > 0 && c->n <= (int64_t)kSkinIdxMask && kind && !bonded && !thermo &&
+           c->pair_kind == KOMD_PAIR_LJ12_6 && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
 }
 
 // One step with Verlet lists, parity par (Simulation.cpp:49-74 with the pair list of
 // LinkedCells::updatePairList rebuilt only when needed, PairList.h:22-24): Step0 and the
 // refreshed FP32 positions; on a rebuild step the container update, binning and the list
-// pass (the other kernels return at once); LJ + ATM from the lists with Step1 fused.
-hipError_t enqueue_skin_step(komd_ctx* c, int par) {
+// pass (the other kernels return at once).  Exact ATM: LJ + ATM from the lists with Step1
+// fused.  Learned and hybrid kinds: the in-range rows and LJ from the lists, then their own
+// three-body passes, the last of which applies Step1 and decides the next rebuild.
+hipError_t enqueue_skin_step(komd_ctx* c, int kind3b, int par) {
     KOMD_CHECK_HIP(launch_skin_drift(c, par));
     KOMD_CHECK_HIP(launch_bin(c, c->d.skin_req + par, c->d.spb));
-    KOMD_CHECK_HIP(launch_skin_list(c, par));
-    c->kick_in = KICK_IN_ATM;
-    const hipError_t e = launch_atm(c, 1, false, nullptr, nullptr, nullptr, true, par);
+    const bool exact = kind3b == KOMD_3B_ATM_EXACT;
+    KOMD_CHECK_HIP(launch_skin_list(c, par, !exact));
+    hipError_t e;
+    if (exact) {
+        c->kick_in = KICK_IN_ATM;
+        e = launch_atm(c, 1, false, nullptr, nullptr, nullptr, true, par);
+    } else {
+        KOMD_CHECK_HIP(launch_skin_refresh(c));
+        c->kick_in = KICK_IN_APPLY;
+        c->skin_step_par = par;
+        e = kind3b == KOMD_3B_SURROGATE ? launch_surrogate_forces(c, false, 0) : hybrid_forces(c, false);
+    }
     c->kick_in = KICK_NONE;
+    c->skin_step_par = -1;
     return e;
 }
 
@@ -823,14 +838,16 @@ struct FlagWatch {
 // parities alternate between two captured graphs (each reads its own rebuild word and
 // clears the other's).  The run ends with a container update without force reset, so the
 // positions are wrapped into the box and the grid matches them, as after a plain run.
-komd_status run_skin(komd_ctx* c, int64_t nsteps) {
-    HIPCALL(c, ensure_skin(c), "komd_run");
+komd_status run_skin(komd_ctx* c, int64_t nsteps, int kind3b) {
+    HIPCALL(c, ensure_skin(c, kind3b != KOMD_3B_ATM_EXACT), "komd_run");
     HIPCALL(c, launch_skin_offsets(c), "komd_run");
     HIPCALL(c, upload_skin_args(c), "komd_run");
     HIPCALL(c, launch_skin_arm(c, c->skin_par), "komd_run");
     if (c->timing || nsteps < 3) {
         for (int64_t k = 0; k < nsteps; k++) {
-            HIPCALL(c, enqueue_skin_step(c, c->skin_par), "komd_run");
+            hipError_t e = enqueue_skin_step(c, kind3b, c->skin_par);
+            if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+            HIPCALL(c, e, "komd_run");
             c->skin_par ^= 1;
             if (c->timing && (k % 64) == 63) {
                 komd_status s = sync_check(c, "komd_run");
@@ -838,14 +855,18 @@ komd_status run_skin(komd_ctx* c, int64_t nsteps) {
             }
         }
     } else {
-        const int key = 1 << 20;
+        const int key = (1 << 20) + kind3b;
         if (!c->graph_exec || !c->graph_exec_odd || c->graph_key != key) {
             drop_graph(c);
             for (int par = 0; par < 2; par++) {
                 hipGraph_t graph = nullptr;
                 HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "komd_run capture");
-                hipError_t e = enqueue_skin_step(c, par);
+                hipError_t e = enqueue_skin_step(c, kind3b, par);
                 hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+                if (e == hipErrorNotSupported) {
+                    if (graph) hipGraphDestroy(graph);
+                    return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
+                }
                 HIPCALL(c, e, "komd_run capture");
                 HIPCALL(c, e2, "komd_run capture");
                 e = hipGraphInstantiate(par ? &c->graph_exec_odd : &c->graph_exec, graph, nullptr, nullptr, 0);
@@ -873,7 +894,6 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
     komd_status s = KOMD_OK;
     const bool bonded = use_bonded && (c->nbonds + c->nangles) > 0;
     const bool thermo = t0 > 0;
-    if (skin_run(c, kind3b, bonded, thermo)) return run_skin(c, nsteps);
     // beta = 1 for the first kick-drift and the target the captured step reads, stream-ordered
     // ahead of the steps (the graph holds no host value of t0, so one graph serves every target)
     if (thermo) HIPCALL(c, launch_run_scalars(c, t0), "komd_run");
@@ -887,6 +907,7 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
             HIPCALL(c, e, "komd_run");
         }
     }
+    if (skin_run(c, kind3b, bonded, thermo)) return run_skin(c, nsteps, kind3b);
     if (c->timing || nsteps < 3) {
         for (int64_t k = 0; k < nsteps; k++) {
             hipError_t e = enqueue_step(c, kind3b, bonded, thermo);

@@ -19,6 +19,37 @@ constexpr uint32_t kSkinIdxMask = (1u << kSkinIdxBits) - 1u;
 constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row
 constexpr int kSurM = 640;         // staged neighbourhood capacity of the surrogate force pass
 
+// Buffers of the Verlet-list passes (one copy per context in device memory, read where they
+// are used: as kernel arguments they would stay in scalar registers through a whole kernel).
+struct SkinArgs {
+    const uint32_t* vl;        // [N][kSkinCap] entries: cell-sorted slot | offset class << 25
+    const uint32_t* vlc;       // [N] lengths
+    const uint16_t* vld;       // [N][kSkinCap] cell direction | rank in that cell << 5 (or null)
+    const float4* spos;        // [N] FP32 positions relative to the build-time cell centres
+    const float4* spb;         // [N] the same at the last rebuild
+    const float4* skin_off;    // [125] offset of each offset class, image code in .w
+    uint32_t* req;             // [2] rebuild words of the two step parities
+    unsigned long long* pair_counters;  // in-range unique pairs per workgroup slot
+    uint32_t* flags;
+    double dt, lim2;           // time step; squared displacement that triggers a rebuild
+};
+
+// Would the next Integrator::Step0 (Integrator.cpp:28-32) move site i more than sqrt(lim2)
+// from its build position?  The step (v + f dt/(2m)) dt of the drift, after this step's
+// second half-kick has been written to v, plus the displacement since the rebuild from the
+// FP32 cell-relative positions (both relative to the same build-time cell centre; their
+// rounding, below 3e-7, lies inside the margin of lim2).
+__device__ __forceinline__ bool skin_exceeded(const double* __restrict__ vx, const double* __restrict__ vy,
+                                              const double* __restrict__ vz, const double* __restrict__ mass,
+                                              uint32_t i, double dt_half, double dt, double fx, double fy, double fz,
+                                              float4 q, float4 qb, double lim2) {
+    const double k = dt_half / mass[i];
+    const double x = (double)(q.x - qb.x) + (vx[i] + fx * k) * dt;
+    const double y = (double)(q.y - qb.y) + (vy[i] + fy * k) * dt;
+    const double z = (double)(q.z - qb.z) + (vz[i] + fz * k) * dt;
+    return !(x * x + y * y + z * z <= lim2);
+}
+
 // ---- helpers shared by the list-driven kernels (k_cell.hip, k_table.hip)
 
 __device__ __forceinline__ double image_shift(uint32_t code, double L) {

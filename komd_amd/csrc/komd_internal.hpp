@@ -86,6 +86,7 @@ struct DevState {
     // last rebuild, entry = cell-sorted slot of the neighbour | offset class of its cell << 25
     uint32_t* vl = nullptr;       // [N][kSkinCap]
     uint32_t* vlc = nullptr;      // [N] lengths
+    uint16_t* vld = nullptr;      // [N][kSkinCap] cell direction | rank << 5 (learned and hybrid kinds)
     float4* spb = nullptr;        // [N] cell-sorted FP32 positions of the last rebuild
     float4* skin_off = nullptr;   // [125] FP32 offset (and image code) of each list offset class
     SkinArgs* skin_args = nullptr;  // the Verlet-list force pass's buffer table (device copy)
@@ -167,6 +168,7 @@ struct komd_ctx {
     // are needed there): 0 none (k_kick runs), else the kernel that applies it
     int kick_in = 0;
     bool lj_in_desc = false;  // set by the NVE step's force pass: the descriptor pass also sums LJ
+    int skin_step_par = -1;   // parity of the Verlet-list step being queued (-1: none)
     int hyb_axis = 0;
     double hyb_center = 0, hyb_half = 0, hyb_blend = 0;
     // exact-ATM judge of surrogate runs
@@ -259,7 +261,11 @@ hipError_t launch_cell(komd_ctx* c, bool lj, int nl, bool energy, bool limit, do
 // skin_par >= 0: a Verlet-list step of that parity (launch_skin_list built or kept the lists)
 hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const float* dlam, double* esite,
                       bool lj = false, int skin_par = -1);
-hipError_t launch_skin_list(komd_ctx* c, int par);
+// dirs: also the cell direction and rank of every entry (d.vld), for launch_skin_refresh
+hipError_t launch_skin_list(komd_ctx* c, int par, bool dirs);
+// the in-range neighbour-list rows (d.nl, d.nlc, d.nld) of every site from its Verlet list
+// at the current positions, and the LJ 12-6 force written as the site's force
+hipError_t launch_skin_refresh(komd_ctx* c);
 hipError_t launch_skin_offsets(komd_ctx* c);  // d.skin_off from the grid
 hipError_t upload_skin_args(komd_ctx* c);     // d.skin_args (before a skinned run)
 size_t skin_args_bytes();
