@@ -142,9 +142,10 @@ def learned_leg(n_atoms, args):
     """Exact ATM and the surrogate on the same equilibrated state of an argon system."""
     import komd_amd as K
     from komd_amd import systems as S
-    sysm = S.config_atm_32000() if n_atoms == 32000 else S.config_atm_256000()
+    cfg = S.config_atm_32000 if n_atoms == 32000 else S.config_atm_256000
+    sysm = cfg(list_radius=2.5 + args.skin)
     n = sysm.n
-    out = {"n_atoms": n}
+    out = {"n_atoms": n, "skin": args.skin}
     with K.Context(sysm, cutoff=2.5, dt=0.005, nu=NU) as ctx:
         ctx.surrogate_set_params(np.load(WEIGHTS))
         ctx.compute_forces(K.ATM_EXACT)
@@ -220,6 +221,8 @@ def learned_leg(n_atoms, args):
         ctx.select_pair(K.PAIR_LJ12_6)
         out["comparator_throughput"] = comp
         legs = {}
+        if args.skin > 0:  # both timed legs on Verlet lists rebuilt on demand
+            ctx.set_skin(args.skin)
         for kind, name in ((K.ATM_EXACT, "exact"), (K.SURROGATE, "surrogate")):
             ctx.set_state(r0, v0)
             ctx.compute_forces(kind)
@@ -235,14 +238,16 @@ def learned_leg(n_atoms, args):
             ctx.timing(True)
             ctx.run(min(args.steps, 200), kind3b=kind)
             kern = {}
-            for k in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_nl", "cell_lj_nl", "atm", "sur_desc",
-                      "sur_mlp", "sur_force", "sur_apply"):
+            for k in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_nl", "cell_lj_nl", "skin_refresh", "atm",
+                      "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
                 ms, cnt = ctx.timer(k)
                 if cnt:
                     kern[k] = round(ms / cnt * 1e3, 2)
             ctx.timing(False)
             legs[name] = {"value": round(n * args.steps / dt, 1), "ms_per_step": round(dt / args.steps * 1e3, 4),
-                          "kernels_us": kern, "binning_GBps": hbm_rates(kern, n)}
+                          "kernels_us": kern}
+            if args.skin <= 0:  # with Verlet lists the binning kernels return at once on most steps
+                legs[name]["binning_GBps"] = hbm_rates(kern, n)
         out["exact"], out["surrogate"] = legs["exact"], legs["surrogate"]
         out["surrogate"]["judge"] = dict(judge, every=JUDGE_EVERY, max_rel_rms=JUDGE_MAX,
                                          note="exact ATM on the current state every 100 steps inside the "
@@ -253,13 +258,13 @@ def learned_leg(n_atoms, args):
             "exact": legs["exact"]["kernels_us"].get("atm"),
             "surrogate": round(sum(legs["surrogate"]["kernels_us"].get(k, 0.0)
                                    for k in ("sur_desc", "sur_mlp", "sur_force", "sur_apply")), 2),
-            "note": "the exact path's k_atm also sums LJ over the lists (its pair pass only builds them); "
-                    "the surrogate path keeps LJ in its pair pass: force_part_us compares like with like"}
+            "note": "the exact path's k_atm also sums LJ over the lists; the surrogate path sums LJ in its "
+                    "descriptor pass (or, on Verlet lists, in skin_refresh): force_part_us compares like with like"}
         # pair pass + three-body kernels: the whole force part of a step but binning
         out["force_part_us"] = {
             name: round(sum(legs[name]["kernels_us"].get(k, 0.0)
-                            for k in ("cell_nl", "cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force",
-                                      "sur_apply")), 2) for name in ("exact", "surrogate")}
+                            for k in ("cell_nl", "cell_lj_nl", "skin_refresh", "atm", "sur_desc", "sur_mlp",
+                                      "sur_force", "sur_apply")), 2) for name in ("exact", "surrogate")}
         if args.drift_steps > 0 and n_atoms == 32000:
             ctx.set_state(r0, v0)
             ctx.compute_forces(K.SURROGATE)
@@ -301,7 +306,7 @@ def hybrid_leg(args):
     the reference.  Throughput counts the timed segments only."""
     import komd_amd as K
     from komd_amd import systems as S
-    sysm = S.config_slab_1048576()
+    sysm = S.config_slab_1048576(list_radius=2.5 + args.skin)
     n = sysm.n
     Lx = float(sysm.box[0])
     center, half, blend = float(sysm.low[0]) + 0.5 * Lx, 0.1 * Lx, 2.0
@@ -322,6 +327,9 @@ def hybrid_leg(args):
         ctx.compute_forces(K.HYBRID)
         ctx.run(200, kind3b=K.HYBRID, t0=0.8)   # melt the lattice
         r0, v0, _ = ctx.get_state()
+        if args.skin > 0:  # the NVE segments on Verlet lists rebuilt on demand
+            ctx.set_skin(args.skin)
+        out["skin"] = args.skin
         legs = {}
         for kind, name in ((K.HYBRID, "hybrid"), (K.ATM_EXACT, "exact_everywhere")):
             ctx.set_state(r0, v0)
@@ -363,7 +371,7 @@ def hybrid_leg(args):
             ctx.timing(True)
             ctx.run(20, kind3b=kind)
             kern = {}
-            for kk in ("cell_lj_nl", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
+            for kk in ("cell_lj_nl", "cell_nl", "skin_refresh", "atm", "sur_desc", "sur_mlp", "sur_force", "sur_apply"):
                 ms, cnt = ctx.timer(kk)
                 if cnt:
                     kern[kk] = round(ms / cnt * 1e3, 2)

@@ -181,8 +181,9 @@ __device__ __forceinline__ void radial_gauss(float y, float g[NR]) {
 // lanes per site of the per-site kernels (neighbours strided over the lanes, sums
 // combined with shuffles inside the lane group)
 constexpr int kLps = 4;
-// Sites of the 27-cell neighbourhood held by the surrogate force pass (about 394 at
-// density 0.8; a larger neighbourhood is reported as a cell overflow).
+// Sites of the 27-cell neighbourhood held by the surrogate force pass (kSurM: on average
+// about 394 at density 0.8 with cells of edge 2.63, 500 with the 2.85 cells of a 0.3 skin;
+// a larger neighbourhood is reported as a cell overflow).
 
 // ---------------------------------------------------------------------------
 // descriptors

@@ -17,7 +17,7 @@ constexpr int kSkinCap = 128;      // Verlet list capacity per site (within cuto
 constexpr uint32_t kSkinIdxBits = 25;
 constexpr uint32_t kSkinIdxMask = (1u << kSkinIdxBits) - 1u;
 constexpr int kRows = 64;          // sites per cell held by a per-cell partial force row
-constexpr int kSurM = 640;         // staged neighbourhood capacity of the surrogate force pass
+constexpr int kSurM = 896;         // staged neighbourhood capacity of the surrogate force pass
 
 // Buffers of the Verlet-list passes (one copy per context in device memory, read where they
 // are used: as kernel arguments they would stay in scalar registers through a whole kernel).

@@ -225,6 +225,6 @@ def config_atm_256000(seed: int = 1004, list_radius: float = 2.5) -> System:
     return fcc(40, 0.8, 0.8, seed, rc=list_radius)
 
 
-def config_slab_1048576(seed: int = 1005) -> System:
+def config_slab_1048576(seed: int = 1005, list_radius: float = 2.5) -> System:
     """1,048,576 argon atoms, simple cubic 256 x 64 x 64 at rho 0.8 in a 4:1:1 box."""
-    return simple_cubic(256, 64, 64, 0.8, 0.8, seed)
+    return simple_cubic(256, 64, 64, 0.8, 0.8, seed, rc=list_radius)
