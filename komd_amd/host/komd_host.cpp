@@ -47,6 +47,8 @@ bool readFile(const std::string& path, Configuration& config, std::vector<Compon
         else if (var == "write_freq") file >> config.write_freq;
         else if (var == "enable_3b") file >> config.enable_3b;
         else if (var == "enable_3b_direct") file >> config.enable_3b_direct;
+        else if (var == "pair_steps") file >> config.pair_steps;
+        else if (var == "triple_steps") file >> config.triple_steps;
         else if (var == "store_checkpoint") file >> config.storeCheckpoint;
         else if (var == "checkpoint_file") {
             std::string p;
@@ -71,6 +73,7 @@ bool readFile(const std::string& path, Configuration& config, std::vector<Compon
         else if (var == "judge_max") file >> config.judge_max;
         else if (var == "hybrid_region") file >> config.hybrid_axis >> config.hybrid_center >> config.hybrid_half >> config.hybrid_blend;
         else if (var == "thermostat") file >> config.thermostat;
+        else if (var == "skin") file >> config.skin;
         else if (var == "checkpoint_precision") file >> config.checkpoint_precision;
         else if (var == "enable_3b_approx") file >> config.enable_3b_approx;
         else if (var == "pair_table") file >> config.pair_table;
@@ -88,6 +91,10 @@ bool readFile(const std::string& path, Configuration& config, std::vector<Compon
     if (config.checkpoint_files.empty()) return fail("No phasespace data specified");
     if (config.cell_size < config.cutoff) return fail("cell size must be larger or equal to cutoff radius");
     if (config.atm_mode != "reference" && config.atm_mode != "exact") return fail("atm_mode: reference or exact");
+    if (config.pair_steps < 1 || config.triple_steps < 1) return fail("pair_steps and triple_steps must be positive");
+    if (!(config.skin >= 0)) return fail("skin must be non-negative");
+    if (config.skin > 0 && config.cell_size < config.cutoff + config.skin)
+        return fail("cell size must be at least cutoff + skin");
     return true;
 }
 
@@ -502,6 +509,9 @@ void Simulation::runDevice(Registry& reg, uint64_t steps) {
         }
     }
     check(c.ctx(), komd_select_pair(c.ctx(), pair), "Simulation::runDevice");
+    // the reference reuses its pair list for pair_steps steps without a skin; the device loop
+    // rebuilds every step unless the deck gives a skin (Verlet lists, no pair missed)
+    check(c.ctx(), komd_set_skin(c.ctx(), reg.config.skin), "Simulation::runDevice");
     check(c.ctx(), komd_compute_forces(c.ctx(), kind, 0), "Simulation::runDevice");
     if (t0 > 0) {
         reg.temperatureSensor->measure();

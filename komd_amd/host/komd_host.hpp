@@ -59,6 +59,9 @@ struct Configuration {
     // GPU path options (no reference counterpart)
     std::string atm_mode = "reference";  // "reference": ATM_NOLIST semantics; "exact": physical, all triplets
     std::string surrogate_weights;       // three-body surrogate parameters (.f32, KOMD_SUR_NPARAM floats)
+    int pair_steps = 4;                  // reference: pair list reused for this many steps (PairList.h:22-24)
+    int triple_steps = 8;                // reference: triple list likewise
+    double skin = 0.0;                   // device loop: Verlet-list skin (0: lists every step), komd_set_skin
     int64_t judge_every = 0;             // exact-ATM judge interval of surrogate runs
     double judge_max = 0.05;             // fallback threshold (relative RMS force error)
     int hybrid_axis = -1;                // >= 0: hybrid exact/surrogate region along this axis

@@ -56,11 +56,13 @@ int main(int argc, char** argv) {
                 "{\"sites\": %zu, \"molecules\": %llu, \"components\": %zu, \"cutoff\": %.17g, \"cell_size\": %.17g, "
                 "\"delta_t\": %.17g, \"temperature\": %.17g, \"nu_internal\": %.17g, \"limit\": %.17g, "
                 "\"domain_high\": [%.17g, %.17g, %.17g], \"enable_3b\": %d, \"atm_mode\": \"%s\", \"timesteps\": %lld, "
-                "\"first_site\": [%.17g, %.17g, %.17g], \"pair\": \"%s\"}\n",
+                "\"first_site\": [%.17g, %.17g, %.17g], \"pair\": \"%s\", \"pair_steps\": %d, \"triple_steps\": %d, "
+                "\"skin\": %.17g}\n",
                 sites.size(), (unsigned long long)mols, comps.size(), config.cutoff, config.cell_size, config.delta_t,
                 config.temperature, config.energy_3b * 1e9 * kConvJEi, config.limit_factor, config.domainHigh[0],
                 config.domainHigh[1], config.domainHigh[2], (int)config.enable_3b, config.atm_mode.c_str(), steps,
-                sites.size() ? sites.r[0] : 0.0, sites.size() ? sites.r[1] : 0.0, sites.size() ? sites.r[2] : 0.0, pair);
+                sites.size() ? sites.r[0] : 0.0, sites.size() ? sites.r[1] : 0.0, sites.size() ? sites.r[2] : 0.0, pair,
+                config.pair_steps, config.triple_steps, config.skin);
             return 0;
         }
         auto reg = makeRegistry(config);

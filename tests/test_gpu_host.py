@@ -93,6 +93,23 @@ def test_device_loop_matches_reference_loop(tmp_path):
     assert np.abs(ra[:, 4:7] - rb[:, 4:7]).max() < 1e-7
 
 
+def test_device_loop_skin_key(tmp_path):
+    """A deck's skin key puts the device loop (NVE) on Verlet lists: the same trajectory as the
+    per-step lists to the FP32 level of the relative coordinates."""
+    s = S.fcc(6, 0.8, 0.8, 74, jitter=0.05, rc=2.8)   # cells of edge >= cutoff + skin
+    out = []
+    for extra in ("thermostat 0\n", "thermostat 0\nskin 0.3\n"):
+        d = deck_for(tmp_path, s, 30, extra=extra)
+        path = str(tmp_path / f"skin{len(out)}.ps")
+        info = run_md(d, "--device-loop", "--write", path)
+        assert info["loop"] == "device"
+        out.append(np.loadtxt(path))
+    dr = np.abs(out[0][:, 1:4] - out[1][:, 1:4])
+    dr = np.minimum(dr, np.abs(dr - float(s.box[0])))
+    print(f"device loop with and without skin: max |dr| {dr.max():.2e}")
+    assert dr.max() < 1e-5
+
+
 def test_surrogate_deck(tmp_path):
     s = S.fcc(6, 0.8, 0.8, 73, jitter=0.05)
     w = os.path.join(ROOT, "komd_amd", "weights", "atm_surrogate.npy")

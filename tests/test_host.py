@@ -88,11 +88,25 @@ def test_reference_test_ps_multisite(tmp_path):
     (lambda s: s.replace("cell_size 5.0", "cell_size 4.0"), "cell size must be larger or equal to cutoff radius"),
     (lambda s: "\n".join(l for l in s.splitlines() if not l.startswith("checkpoint_file")), "No phasespace data"),
     (lambda s: s.replace("COMP 0", "COMP -1"), "negative component id"),
+    (lambda s: s + "pair_steps 0\n", "pair_steps and triple_steps must be positive"),
+    (lambda s: s + "skin 0.5\n", "cell size must be at least cutoff + skin"),
+    (lambda s: s + "skin -1\n", "skin must be non-negative"),
 ])
 def test_deck_errors(tmp_path, mutation, message):
     d = deck(tmp_path, mutation(IBI_DECK.format(ps=IBI_PS)))
     res = run(d, "--check", check=False)
     assert res.returncode == 1 and message in res.stderr
+
+
+def test_pair_list_keys(tmp_path):
+    """The reference's pair_steps / triple_steps keys (FileInput.cpp:36-37, defaults 4 / 8 in
+    util/defaults.h:39-40) are read; the device loop's Verlet-list skin is its own key."""
+    d = deck(tmp_path, IBI_DECK.format(ps=IBI_PS).replace("cell_size 5.0", "cell_size 5.5")
+             + "pair_steps 2\ntriple_steps 3\nskin 0.5\n")
+    out = json.loads(run(d, "--check").stdout)
+    assert (out["pair_steps"], out["triple_steps"], out["skin"]) == (2, 3, 0.5)
+    out = json.loads(run(deck(tmp_path, IBI_DECK.format(ps=IBI_PS)), "--check").stdout)
+    assert (out["pair_steps"], out["triple_steps"], out["skin"]) == (4, 8, 0.0)
 
 
 @pytest.mark.parametrize("extra,pair", [("", "lj12_6"), ("enable_3b_approx 1\n", "atm2b"),
