@@ -90,6 +90,8 @@ def parse():
                     help="seeds of the exact / surrogate drift comparison at 32,000 atoms (0: skip)")
     ap.add_argument("--structure-steps", type=int, default=1000,
                     help="NVE steps of the g(r) / pressure comparison, surrogate against exact (0: skip)")
+    ap.add_argument("--only-two-body", action="store_true",
+                    help="only run the LJ-4000 and chains-262144 legs and print their JSON (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -451,7 +453,9 @@ def other_legs(args):
         out["lj_4000"] = {"workload": "4,000 LJ argon atoms, FCC 10^3 x 4, rho 0.8, T* 1.0, rc 2.5, dt 0.005, NVE",
                           "steps": 1000, "value": round(s1.n * 1000 / dt, 1), "ms_per_step": round(dt, 4),
                           "energy_per_atom_start": e0, "energy_per_atom_end": e1}
-    s2 = S.config_chains_262144()
+    # cells of edge >= cutoff + skin for the Verlet-list run (the per-step run below uses its
+    # own grid of cells >= cutoff)
+    s2 = S.config_chains_262144(list_radius=2.5 + max(args.skin, 0.0))
     topo = {"bonds": s2.bonds, "angles": s2.angles}
     # capped-force warm-up from the lattice (the clamp would also cap the stiff bonds later)
     # sites binned on their own positions: a chain reaches ~1.2 sigma from its centre, so
@@ -463,31 +467,56 @@ def other_legs(args):
             ctx.compute_forces(0, bonded=True)
             ctx.run(300, kind3b=0, bonded=True, t0=1.0)
             s2.r, s2.v, _ = ctx.get_state()
-    with K.Context(s2, cutoff=2.5, dt=0.005, topology=topo, bin_mode=K.BIN_SITE) as ctx:
-        ctx.compute_forces(0, bonded=True)
-        ctx.run(args.warmup, kind3b=0, bonded=True, t0=1.0)
-        t0 = time.perf_counter()
-        ctx.run(args.steps, kind3b=0, bonded=True)
-        dt = time.perf_counter() - t0
-        ctx.timing(True)
-        ctx.run(100, kind3b=0, bonded=True)
-        kern = {}
-        for k in ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj", "bonded", "kick"):
-            ms, cnt = ctx.timer(k)
-            if cnt:
-                kern[k] = round(ms / cnt * 1e3, 2)
-        ctx.timing(False)
-        obs = ctx.observables(0)
-        out["chains_262144"] = {
-            "workload": "65,536 linear 4-site chains (262,144 sites), site density 0.7, harmonic bonds k 500 r0 1, "
-                        "angles k 50 theta0 109.5 deg, LJ rc 2.5 between molecules, sites binned on their own "
-                        "positions, dt 0.005, NVE after a warm-up with the force cap raised from 20 to 2e5 "
-                        "in five 300-step stages at dt 0.001 with rescaling to T* 1.0",
-            "steps": args.steps, "value": round(s2.n * args.steps / dt, 1),
-            "ms_per_step": round(dt / args.steps * 1e3, 4), "kernels_us": kern,
-            "u_bond_per_site": obs["u_bond"] / s2.n, "u_angle_per_site": obs["u_angle"] / s2.n,
-            "temperature": obs["temperature"],
-            "equipartition": "3 bonds and 2 angles per 4 sites: kT/2 each gives 0.375 T and 0.25 T per site"}
+    chain_kernels = ("bin_key", "bin_scan", "bin_scatter", "bin_sort", "cell_lj", "cell_nl", "bonded", "skin_lj",
+                     "kick")
+
+    def chain_leg(sysc, skin):
+        """Rescaled warm-up steps, then the timed NVE steps and 100 steps under kernel timers."""
+        with K.Context(sysc, cutoff=2.5, dt=0.005, topology=topo, bin_mode=K.BIN_SITE) as ctx:
+            ctx.compute_forces(0, bonded=True)
+            ctx.run(args.warmup, kind3b=0, bonded=True, t0=1.0)
+            if skin > 0:
+                ctx.set_skin(skin)
+                ctx.run(args.warmup, kind3b=0, bonded=True)  # the Verlet-list graphs, captured
+            r0 = ctx.skin_rebuilds()
+            t0 = time.perf_counter()
+            ctx.run(args.steps, kind3b=0, bonded=True)
+            dt = time.perf_counter() - t0
+            reb = ctx.skin_rebuilds() - r0
+            ctx.timing(True)
+            ctx.run(100, kind3b=0, bonded=True)
+            kern = {}
+            for k in chain_kernels:
+                ms, cnt = ctx.timer(k)
+                if cnt:
+                    kern[k] = round(ms / cnt * 1e3, 2)
+            ctx.timing(False)
+            obs = ctx.observables(0)
+        return dt, reb, kern, obs
+
+    dt, reb, kern, obs = chain_leg(s2, args.skin)
+    out["chains_262144"] = {
+        "workload": "65,536 linear 4-site chains (262,144 sites), site density 0.7, harmonic bonds k 500 r0 1, "
+                    "angles k 50 theta0 109.5 deg, LJ rc 2.5 between molecules, sites binned on their own "
+                    "positions, dt 0.005, NVE after a warm-up with the force cap raised from 20 to 2e5 "
+                    "in five 300-step stages at dt 0.001 with rescaling to T* 1.0",
+        "steps": args.steps, "value": round(s2.n * args.steps / dt, 1),
+        "ms_per_step": round(dt / args.steps * 1e3, 4), "kernels_us": kern,
+        "u_bond_per_site": obs["u_bond"] / s2.n, "u_angle_per_site": obs["u_angle"] / s2.n,
+        "temperature": obs["temperature"],
+        "equipartition": "3 bonds and 2 angles per 4 sites: kT/2 each gives 0.375 T and 0.25 T per site"}
+    if args.skin > 0:
+        out["chains_262144"]["neighbour_lists"] = {
+            "kind": "Verlet lists of sites of other molecules within cutoff + skin, rebuilt on demand",
+            "skin": args.skin, "grid_cells_per_dim": int(np.floor(s2.box[0] / s2.cell_size)),
+            "rebuilds_in_timed_steps": int(reb)}
+        # the same state on a grid of cells >= cutoff, container update every step
+        s3 = s2.copy()
+        s3.cell_size = S.cell_size_for(float(s2.box[0]), 2.5)
+        dtp, _, kernp, _ = chain_leg(s3, 0.0)
+        out["chains_262144"]["per_step_rebuild"] = {
+            "value": round(s3.n * args.steps / dtp, 1), "ms_per_step": round(dtp / args.steps * 1e3, 4),
+            "grid_cells_per_dim": int(np.floor(s3.box[0] / s3.cell_size)), "kernels_us": kernp}
     return out
 
 
@@ -580,6 +609,9 @@ def main():
         sys.exit(2)
     if args.copy_mib:
         copy_calibration(args.copy_mib)
+        return
+    if args.only_two_body:
+        print(json.dumps(other_legs(args)), flush=True)
         return
 
     import komd_amd as K

@@ -236,8 +236,11 @@ komd_status komd_compute_forces(komd_ctx* ctx, int32_t kind3b, int32_t use_bonde
  * decision near the cutoff as a fresh list), and the device rebuilds the container and the
  * lists before any site has moved more than skin/2 since the last rebuild.  Every step's
  * interaction set therefore equals the one of a container update at that step.  Applies to
- * komd_run with KOMD_3B_ATM_EXACT, LJ 12-6 of uniform single-site sites without the clamp,
- * no bonded terms and no thermostat (t0 <= 0); other runs ignore it.  The run's last
+ * komd_run without thermostat (t0 <= 0) with LJ 12-6 of uniform sites without the clamp and
+ * either KOMD_3B_ATM_EXACT, KOMD_3B_SURROGATE or KOMD_3B_HYBRID (single-site, no bonded
+ * terms), or no three-body term: single-site molecules (bonded terms allowed), or multi-site
+ * molecules with KOMD_BIN_SITE and bonded terms inside molecules, use_bonded set (their lists
+ * leave same-molecule pairs out, LJ12_6.cpp:19); other runs ignore it.  The run's last
  * container update wraps the positions and leaves the grid consistent with them.
  * skin = 0 (default) turns it off; skin > 0 needs cell_size >= cutoff + skin. */
 komd_status komd_set_skin(komd_ctx* ctx, double skin);

@@ -334,7 +334,8 @@ class Context:
         self._check(self.lib.komd_run(self._ctx, nsteps, kind3b, 1 if bonded else 0, t0))
 
     def set_skin(self, skin: float):
-        """Verlet lists of width cutoff + skin for komd_run (exact ATM + LJ, NVE); 0: off.
+        """Verlet lists of width cutoff + skin for NVE komd_run with LJ 12-6 (exact ATM,
+        surrogate, hybrid, or no three-body term incl. site-binned bonded chains); 0: off.
         Needs cell_size >= cutoff + skin."""
         self._check(self.lib.komd_set_skin(self._ctx, float(skin)))
 

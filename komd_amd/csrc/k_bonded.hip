@@ -103,6 +103,25 @@ __global__ __launch_bounds__(256) void k_angles(int na, const uint32_t* __restri
     }
 }
 
+// Is the same-molecule pair (i, j), i < j, within the cutoff, decided as the pair pass
+// decides it from the lower-index site: FP64 |(r_i + 0) - (r_j + s_j)|^2 in that order,
+// s_j the periodic image of j nearest i (the image of j's cell seen from i's cell)?
+__device__ __forceinline__ uint32_t intra_in_range(double xi, double yi, double zi, double xj, double yj,
+                                                   double zj, const Grid& g, double rc2) {
+#pragma clang fp contract(off)
+    const double Lx = g.size[0], Ly = g.size[1], Lz = g.size[2];
+    const double ex = xj - xi, ey = yj - yi, ez = zj - zi;
+    const double sx = ex > 0.5 * Lx ? -Lx : (ex < -0.5 * Lx ? Lx : 0.0);
+    const double sy = ey > 0.5 * Ly ? -Ly : (ey < -0.5 * Ly ? Ly : 0.0);
+    const double sz = ez > 0.5 * Lz ? -Lz : (ez < -0.5 * Lz ? Lz : 0.0);
+    const double dx = (xi + 0.0) - (xj + sx), dy = (yi + 0.0) - (yj + sy), dz = (zi + 0.0) - (zj + sz);
+    double d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    return d2 > rc2 ? 0u : 1u;
+}
+constexpr int kMolRegs = 4;
+
 // Bonds and angles grouped by molecule (CSR): one thread per molecule updates the
 // forces of its own sites with plain read-modify-writes (no two threads share a
 // site), in a fixed order.  Same terms and arithmetic as k_bonds / k_angles.
@@ -114,10 +133,36 @@ __global__ __launch_bounds__(256) void k_bonded_mol(int nmol, const uint32_t* __
                                                     const double* __restrict__ ry, const double* __restrict__ rz,
                                                     double* __restrict__ fx, double* __restrict__ fy,
                                                     double* __restrict__ fz, Grid g, double* __restrict__ part,
-                                                    double* __restrict__ vpart, int vparts, int energy) {
+                                                    double* __restrict__ vpart, int vparts, int energy,
+                                                    const uint32_t* __restrict__ mol_start, double rc2,
+                                                    unsigned long long* __restrict__ intra, int nslots) {
     double ub = 0, ua = 0;
     float w[kSym] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t nintra = 0;
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < nmol; m += gridDim.x * blockDim.x) {
+        if (intra) {
+            // Verlet-list runs: the lists hold no same-molecule pairs, so the in-range ones are
+            // counted here (molecules of up to kMolRegs sites from registers)
+            const uint32_t s0 = mol_start[m], s1 = mol_start[m + 1];
+            const int ns = (int)(s1 - s0);
+            if (ns <= kMolRegs) {
+                double x[kMolRegs], y[kMolRegs], z[kMolRegs];
+#pragma unroll
+                for (int u = 0; u < kMolRegs; u++) {
+                    const uint32_t su = u < ns ? s0 + u : s0;
+                    x[u] = rx[su]; y[u] = ry[su]; z[u] = rz[su];
+                }
+#pragma unroll
+                for (int u = 0; u < kMolRegs; u++)
+#pragma unroll
+                    for (int w2 = u + 1; w2 < kMolRegs; w2++)
+                        if (w2 < ns) nintra += intra_in_range(x[u], y[u], z[u], x[w2], y[w2], z[w2], g, rc2);
+            } else {
+                for (uint32_t i = s0; i < s1; i++)
+                    for (uint32_t j = i + 1; j < s1; j++)
+                        nintra += intra_in_range(rx[i], ry[i], rz[i], rx[j], ry[j], rz[j], g, rc2);
+            }
+        }
         for (uint32_t t = bstart[m]; t < bstart[m + 1]; t++) {
             const uint32_t i = bonds[2 * t], j = bonds[2 * t + 1];
             const float dx = min_image(rx[j] - rx[i], g.size[0]);
@@ -170,6 +215,16 @@ __global__ __launch_bounds__(256) void k_bonded_mol(int nmol, const uint32_t* __
             }
         }
     }
+    if (intra) {
+        // one add per workgroup, each to its own slot (same-address atomics serialise)
+        __shared__ uint32_t wsum[4];
+        for (int o = 32; o > 0; o >>= 1) nintra += __shfl_xor(nintra, o);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = nintra;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            atomicAdd(&intra[blockIdx.x % nslots],
+                      (unsigned long long)wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+    }
     if (energy) {
         block_store_sym(vpart, vparts, blockIdx.x, w);
         __shared__ double sh[2][256];
@@ -208,16 +263,20 @@ __global__ __launch_bounds__(256) void k_limit(int n, double lim, const double* 
 
 // Energy mode: U partials in d.red ([0, 256) bonds, [256, 512) angles) and the virial
 // tensor in d.vpart as kSym slots of 512 partials; the caller zeroes both first.
-hipError_t launch_bonded(komd_ctx* c, bool energy) {
+hipError_t launch_bonded(komd_ctx* c, bool energy, bool intra) {
     ScopedTimer tm(c, "bonded");
     auto& d = c->d;
     const int blocks = 256;
+    if (intra && (!c->bonded_by_mol || !d.mol_start || !d.cnt_trip)) return hipErrorInvalidValue;
     if (c->bonded_by_mol) {
         const int nmol = (int)c->nmol;
+        // the Verlet-list passes' first pair-count slot
+        unsigned long long* ic = intra ? d.cnt_trip + atm_blocks(c) : nullptr;
         k_bonded_mol<<<blocks, 256, 0, c->stream>>>(nmol, d.bond_mol, d.angle_mol, d.bonds, d.angles, (float)c->kb,
                                                      (float)c->r0, (float)c->ka, (float)c->theta0, d.rx, d.ry, d.rz,
                                                      d.fx, d.fy, d.fz, c->grid, d.red, d.vpart, 2 * blocks,
-                                                     energy ? 1 : 0);
+                                                     energy ? 1 : 0, d.mol_start,
+                                                     c->prm.cutoff * c->prm.cutoff, ic, atm_blocks(c));
         return hipGetLastError();
     }
     if (c->nbonds > 0) {

@@ -286,7 +286,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cell(CellArgs a) {
                     if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, wi, w) > a.rc2);
                 }
                 const uint32_t oh = in ? oj : oi;  // hit: in range and not i itself
-                const bool hit = oh != oi;
+                // Verlet lists of multi-site molecules hold sites of other molecules only
+                const bool hit = oh != oi && !(VL && MOLS && cmol[t] == mi);
                 if (!VL) npairs += (unsigned long long)__popcll(ballot(oh > oi));  // wave-uniform
                 if (VL) {
                     const unsigned long long bal = ballot(hit) & gmask;
@@ -859,6 +860,103 @@ __global__ __launch_bounds__(64 * kAtmWaves) void k_skin_refresh(RefreshArgs a) 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Verlet-list step without a three-body term (LJ 12-6 of uniform sites, no clamp): each
+// site's in-range partners filtered from its Verlet list at the current positions (the FP32
+// vectors and the FP64 decision near the cutoff of the Verlet-list ATM pass), their LJ force
+// added to the site's force (zero, or the step's bonded terms, which ran first: the order of
+// the per-step run), then Step1 and the next step's rebuild decision.  Lists of multi-site
+// molecules hold sites of other molecules only (the exclusion of LJ12_6.cpp:19 taken at the
+// build).  One 16-lane DPP row per site, four sites per wave; the row sums its FP64 partial
+// forces with quad_perm + row_ror, in a fixed order.
+struct SkinLjArgs {
+    const SkinArgs* __restrict__ sk;
+    const uint32_t* __restrict__ sorted;
+    const double* __restrict__ rx;
+    const double* __restrict__ ry;
+    const double* __restrict__ rz;
+    double* __restrict__ fx;
+    double* __restrict__ fy;
+    double* __restrict__ fz;
+    double* __restrict__ vx;
+    double* __restrict__ vy;
+    double* __restrict__ vz;
+    const double* __restrict__ mass;
+    Grid g;
+    int n, chunk, add, par;
+    float rc2f, band, sig2, eps24;
+    double rc2, kick;
+};
+constexpr int kLjWaves = 4;
+constexpr int kLjSites = 4 * kLjWaves;  // sites per workgroup
+
+__global__ __launch_bounds__(64 * kLjWaves) void k_skin_lj(SkinLjArgs a) {
+    __shared__ float4 woff[125];
+    __shared__ unsigned long long red_np[kLjWaves];
+    const int lane = threadIdx.x & 63, sub = lane & 15;
+    const int wvu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const SkinArgs* sk = a.sk;
+    if (threadIdx.x < 125) woff[threadIdx.x] = sk->skin_off[threadIdx.x];
+    __syncthreads();
+    const int b = blockIdx.x;
+    // XCD-aware site slabs; the four rows of a wave take consecutive sites
+    const int p = (((b & 7) * a.chunk + (b >> 3)) * kLjWaves + wvu) * 4 + (lane >> 4);
+    const bool act = p < a.n;
+    const uint32_t self = act ? (uint32_t)p : 0u;  // dead lanes read slot 0, class 0
+    const float4* spos = sk->spos;
+    const uint32_t oi = act ? a.sorted[p] : 0u;
+    const float4 qi = spos[self];
+    const int nv = act ? (int)sk->vlc[p] : 0;
+    const uint32_t* vrow = sk->vl + (size_t)self * kSkinCap;
+    double ljx = 0.0, ljy = 0.0, ljz = 0.0;
+    unsigned long long npair = 0;
+    for (int t0 = 0; ballot(t0 < nv) != 0ull; t0 += 16) {  // wave-uniform trip count
+        const int s = t0 + sub;
+        const bool live = s < nv;
+        const uint32_t e = live ? vrow[s] : self;
+        const float4 q = spos[e & kSkinIdxMask];
+        const float4 o = woff[e >> kSkinIdxBits];
+        const float dx = (q.x + o.x) - qi.x, dy = (q.y + o.y) - qi.y, dz = (q.z + o.z) - qi.z;
+        const uint32_t wj = __float_as_uint(q.w) | __float_as_uint(o.w);
+        const float r2 = live ? dx * dx + dy * dy + dz * dz : __builtin_huge_valf();
+        bool in = r2 <= a.rc2f;
+        const bool amb = fabsf(r2 - a.rc2f) <= a.band;
+        if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, oi, wj) > a.rc2);
+        npair += (unsigned long long)__popcll(ballot(in && (wj & kIdxMask) > oi));
+        if (in) {
+            // the pair pass's LJ 12-6 expression: force on i = -(r_j - r_i) fac
+            const float iX = __builtin_amdgcn_rcpf(r2);
+            const float s2 = a.sig2 * iX;
+            const float s6 = s2 * s2 * s2;
+            const float fac = a.eps24 * (2.f * s6 * s6 - s6) * iX;
+            ljx -= (double)(dx * fac);
+            ljy -= (double)(dy * fac);
+            ljz -= (double)(dz * fac);
+        }
+    }
+    // the row's sum in every lane of the row
+    ljx += dpp_f64<0xB1>(ljx); ljx += dpp_f64<0x4E>(ljx); ljx += dpp_f64<0x124>(ljx); ljx += dpp_f64<0x128>(ljx);
+    ljy += dpp_f64<0xB1>(ljy); ljy += dpp_f64<0x4E>(ljy); ljy += dpp_f64<0x124>(ljy); ljy += dpp_f64<0x128>(ljy);
+    ljz += dpp_f64<0xB1>(ljz); ljz += dpp_f64<0x4E>(ljz); ljz += dpp_f64<0x124>(ljz); ljz += dpp_f64<0x128>(ljz);
+    if (act && sub == 0) {
+        const double nfx = (a.add ? a.fx[oi] : 0.0) + ljx, nfy = (a.add ? a.fy[oi] : 0.0) + ljy,
+                     nfz = (a.add ? a.fz[oi] : 0.0) + ljz;
+        a.fx[oi] = nfx;
+        a.fy[oi] = nfy;
+        a.fz[oi] = nfz;
+        kick_site(a.vx, a.vy, a.vz, a.mass, oi, a.kick, nfx, nfy, nfz);
+        if (skin_exceeded(a.vx, a.vy, a.vz, a.mass, oi, a.kick, sk->dt, nfx, nfy, nfz, qi, sk->spb[p], sk->lim2))
+            atomicOr(sk->req + (a.par ^ 1), 1u);
+    }
+    if (lane == 0) red_np[wvu] = npair;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tp = 0;
+        for (int k = 0; k < kLjWaves; k++) tp += red_np[k];
+        atomicAdd(&sk->pair_counters[b], tp);
+    }
+}
+
 template <bool U, bool M, bool LJ, int NL, bool E, bool LIM>
 void go(komd_ctx* c, const CellArgs& a) {
     k_cell<U, M, LJ, NL, E, LIM><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
@@ -1000,15 +1098,46 @@ hipError_t launch_skin_list(komd_ctx* c, int par, bool dirs) {
     a.rl2f = (float)(rl * rl);
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;
-    if (!d.vl || !d.vlc || !d.skin_req || !c->uniform || c->multisite || c->n > (int64_t)kSkinIdxMask)
+    if (!d.vl || !d.vlc || !d.skin_req || !c->uniform || c->n > (int64_t)kSkinIdxMask ||
+        (c->multisite && (dirs || !d.sid)))
         return hipErrorInvalidValue;
     ScopedTimer tm(c, "cell_nl");
-    if (dirs) {
+    if (c->multisite) {
+        k_cell<true, true, false, 3, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
+    } else if (dirs) {
         if (!d.vld) return hipErrorInvalidValue;
         k_cell<true, false, false, 4, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
     } else {
         k_cell<true, false, false, 3, false, false><<<a.nblocks, kCellThreads, 0, c->stream>>>(a);
     }
+    return hipGetLastError();
+}
+
+// LJ from the Verlet lists for a skinned step of parity par without a three-body term; add:
+// the step's bonded forces are already in f (else f is written outright).
+hipError_t launch_skin_lj(komd_ctx* c, int par, bool add) {
+    auto& d = c->d;
+    if (!d.skin_args || !c->uniform || c->n > (int64_t)kSkinIdxMask) return hipErrorInvalidValue;
+    SkinLjArgs a{};
+    a.sk = d.skin_args;
+    a.sorted = d.sorted; a.rx = d.rx; a.ry = d.ry; a.rz = d.rz;
+    a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
+    a.vx = d.vx; a.vy = d.vy; a.vz = d.vz; a.mass = d.mass;
+    a.g = c->grid;
+    a.n = (int)c->n;
+    a.add = add ? 1 : 0;
+    a.par = par;
+    // at most atm_blocks workgroups: the pair-count slots of the Verlet-list passes
+    const int blocks = 8 * ((((int)c->n + kLjSites - 1) / kLjSites + 7) / 8);
+    a.chunk = blocks / 8;
+    a.rc2 = c->prm.cutoff * c->prm.cutoff;
+    a.rc2f = (float)a.rc2;
+    a.band = 4e-6f * a.rc2f;
+    a.sig2 = c->sig_u * c->sig_u;
+    a.eps24 = 24.f * c->eps_u;
+    a.kick = 0.5 * c->prm.delta_t;
+    ScopedTimer tm(c, "skin_lj");
+    k_skin_lj<<<blocks, 64 * kLjWaves, 0, c->stream>>>(a);
     return hipGetLastError();
 }
 

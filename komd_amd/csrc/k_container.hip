@@ -127,11 +127,13 @@ __global__ __launch_bounds__(kBlock) void k_com_wrap_key(
     bin_unit(s0, s1, rx, ry, rz, fx, fy, fz, mass, key, slot, count, g, bin_site, reset_f, flags);
 }
 
-// Step0 of a Verlet-list run (single-site molecules).  req[0] != 0 marks a rebuild step:
+// Step0 of a Verlet-list run (single-site molecules, or sites binned on their own
+// positions).  req[0] != 0 marks a rebuild step:
 // then the container update follows as in k_com_wrap_key.  Otherwise the site keeps its
 // build-time cell and
 // no wrap happens (the lists carry the image offsets of the build); only its FP32
 // position relative to that cell's centre is refreshed, as k_sort_repack computes it.
+// zero_f: the force is zeroed on every step (the step's first force kernel adds to it).
 // Thread 0 clears the request word of the other step parity (the step's last force kernel
 // sets it when some site would move beyond half the skin at the next drift) and counts
 // the rebuilds.  words: [0], [1] the rebuild words of the two parities, [2] rebuilds.
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_skin_drift(
     double* __restrict__ fz, const double* __restrict__ mass, uint32_t* __restrict__ key,
     uint32_t* __restrict__ slot, uint32_t* __restrict__ count, Grid g, int bin_site, double dt, double dt_half,
     uint32_t* __restrict__ words, int par, float4* __restrict__ spos,
-    const uint32_t* __restrict__ inv, uint32_t* __restrict__ flags) {
+    const uint32_t* __restrict__ inv, int zero_f, uint32_t* __restrict__ flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool rebuild = words[par] != 0u;
     if (i == 0) {
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_skin_drift(
         bin_unit(i, i + 1, rx, ry, rz, fx, fy, fz, mass, key, slot, count, g, bin_site, 1, flags);
         return;
     }
+    if (zero_f) { fx[i] = 0.0; fy[i] = 0.0; fz[i] = 0.0; }  // the step's forces add onto zero
     const double x = rx[i], y = ry[i], z = rz[i];
     if (!(x - x == 0.0) || !(y - y == 0.0) || !(z - z == 0.0)) atomicOr(flags, FLAG_NONFINITE);
     const uint32_t k = key[i];
@@ -487,14 +490,15 @@ hipError_t launch_com_wrap_key_beta(komd_ctx* c, const double* beta_dev) {
     return hipGetLastError();
 }
 
-hipError_t launch_skin_drift(komd_ctx* c, int par) {
+hipError_t launch_skin_drift(komd_ctx* c, int par, bool zero_f) {
     ScopedTimer tm(c, "bin_key");
     auto& d = c->d;
     const int n = (int)c->n;
+    if (c->multisite && c->prm.bin_mode != KOMD_BIN_SITE) return hipErrorInvalidValue;
     k_skin_drift<<<(n + kKeyBlock - 1) / kKeyBlock, kKeyBlock, 0, c->stream>>>(
         n, d.rx, d.ry, d.rz, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.key, d.slot, d.count, c->grid,
         c->prm.bin_mode == KOMD_BIN_SITE ? 1 : 0, c->prm.delta_t, c->prm.delta_t * 0.5, d.skin_req, par, d.spos,
-        d.inv, d.flags);
+        d.inv, zero_f ? 1 : 0, d.flags);
     return hipGetLastError();
 }
 

@@ -284,12 +284,18 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
     return hipSuccess;
 }
 
-// Runs that use Verlet lists (komd_set_skin): the exact ATM, the surrogate or the hybrid slab
-// with LJ 12-6 of uniform single-site sites without the clamp, NVE, no bonded terms.
+// Runs that use Verlet lists (komd_set_skin): NVE with LJ 12-6 of uniform sites without the
+// clamp, and either the exact ATM, the surrogate or the hybrid slab (single-site, no bonded
+// terms), or no three-body term: single-site sites (bonded terms allowed), or multi-site
+// molecules binned by site whose bonded terms are grouped by molecule (that kernel counts
+// the same-molecule pairs the lists leave out).
 bool skin_run(const komd_ctx* c, int kind3b, bool bonded, bool thermo) {
-    const bool kind = kind3b == KOMD_3B_ATM_EXACT || kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID;
-    return c->skin > 0 && c->n <= (int64_t)kSkinIdxMask && kind && !bonded && !thermo &&
-           c->pair_kind == KOMD_PAIR_LJ12_6 && c->uniform && !c->multisite && !(c->prm.limit_factor > 0);
+    const bool three = kind3b == KOMD_3B_ATM_EXACT || kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID;
+    const bool two = kind3b == 0 &&
+                     (!c->multisite || (c->prm.bin_mode == KOMD_BIN_SITE && bonded && c->bonded_by_mol));
+    const bool sites = three ? (!bonded && !c->multisite) : two;
+    return c->skin > 0 && c->n <= (int64_t)kSkinIdxMask && sites && !thermo &&
+           c->pair_kind == KOMD_PAIR_LJ12_6 && c->uniform && !(c->prm.limit_factor > 0);
 }
 
 // One step with Verlet lists, parity par (Simulation.cpp:49-74 with the pair list of
@@ -297,12 +303,19 @@ bool skin_run(const komd_ctx* c, int kind3b, bool bonded, bool thermo) {
 // refreshed FP32 positions; on a rebuild step the container update, binning and the list
 // pass (the other kernels return at once).  Exact ATM: LJ + ATM from the lists with Step1
 // fused.  Learned and hybrid kinds: the in-range rows and LJ from the lists, then their own
-// three-body passes, the last of which applies Step1 and decides the next rebuild.
-hipError_t enqueue_skin_step(komd_ctx* c, int kind3b, int par) {
-    KOMD_CHECK_HIP(launch_skin_drift(c, par));
+// three-body passes, the last of which applies Step1 and decides the next rebuild.  No
+// three-body term: the bonded terms (FENE's slot, Simulation.cpp:55-56), then LJ from the
+// lists added to them with Step1 fused.
+hipError_t enqueue_skin_step(komd_ctx* c, int kind3b, int par, bool bonded) {
+    const bool two = kind3b == 0;
+    KOMD_CHECK_HIP(launch_skin_drift(c, par, two && bonded));
     KOMD_CHECK_HIP(launch_bin(c, c->d.skin_req + par, c->d.spb));
     const bool exact = kind3b == KOMD_3B_ATM_EXACT;
-    KOMD_CHECK_HIP(launch_skin_list(c, par, !exact));
+    KOMD_CHECK_HIP(launch_skin_list(c, par, !exact && !two));
+    if (two) {
+        if (bonded) KOMD_CHECK_HIP(launch_bonded(c, false, c->multisite));
+        return launch_skin_lj(c, par, bonded);
+    }
     hipError_t e;
     if (exact) {
         c->kick_in = KICK_IN_ATM;
@@ -838,14 +851,14 @@ struct FlagWatch {
 // parities alternate between two captured graphs (each reads its own rebuild word and
 // clears the other's).  The run ends with a container update without force reset, so the
 // positions are wrapped into the box and the grid matches them, as after a plain run.
-komd_status run_skin(komd_ctx* c, int64_t nsteps, int kind3b) {
-    HIPCALL(c, ensure_skin(c, kind3b != KOMD_3B_ATM_EXACT), "komd_run");
+komd_status run_skin(komd_ctx* c, int64_t nsteps, int kind3b, bool bonded) {
+    HIPCALL(c, ensure_skin(c, kind3b != KOMD_3B_ATM_EXACT && kind3b != 0), "komd_run");
     HIPCALL(c, launch_skin_offsets(c), "komd_run");
     HIPCALL(c, upload_skin_args(c), "komd_run");
     HIPCALL(c, launch_skin_arm(c, c->skin_par), "komd_run");
     if (c->timing || nsteps < 3) {
         for (int64_t k = 0; k < nsteps; k++) {
-            hipError_t e = enqueue_skin_step(c, kind3b, c->skin_par);
+            hipError_t e = enqueue_skin_step(c, kind3b, c->skin_par, bonded);
             if (e == hipErrorNotSupported) return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
             HIPCALL(c, e, "komd_run");
             c->skin_par ^= 1;
@@ -855,13 +868,13 @@ komd_status run_skin(komd_ctx* c, int64_t nsteps, int kind3b) {
             }
         }
     } else {
-        const int key = (1 << 20) + kind3b;
+        const int key = (1 << 20) + (bonded ? 1 << 16 : 0) + kind3b;
         if (!c->graph_exec || !c->graph_exec_odd || c->graph_key != key) {
             drop_graph(c);
             for (int par = 0; par < 2; par++) {
                 hipGraph_t graph = nullptr;
                 HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "komd_run capture");
-                hipError_t e = enqueue_skin_step(c, kind3b, par);
+                hipError_t e = enqueue_skin_step(c, kind3b, par, bonded);
                 hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
                 if (e == hipErrorNotSupported) {
                     if (graph) hipGraphDestroy(graph);
@@ -907,7 +920,7 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
             HIPCALL(c, e, "komd_run");
         }
     }
-    if (skin_run(c, kind3b, bonded, thermo)) return run_skin(c, nsteps, kind3b);
+    if (skin_run(c, kind3b, bonded, thermo)) return run_skin(c, nsteps, kind3b, bonded);
     if (c->timing || nsteps < 3) {
         for (int64_t k = 0; k < nsteps; k++) {
             hipError_t e = enqueue_step(c, kind3b, bonded, thermo);

@@ -215,7 +215,7 @@ hipError_t launch_com_wrap_key(komd_ctx* c, bool do_drift, bool reset_f = true);
 hipError_t launch_bin(komd_ctx* c, const uint32_t* req = nullptr, float4* spb = nullptr);
 // Verlet-list runs: Step0 + (rebuild steps) container update, else the FP32 position
 // refresh; par is the step parity whose rebuild word is read
-hipError_t launch_skin_drift(komd_ctx* c, int par);
+hipError_t launch_skin_drift(komd_ctx* c, int par, bool zero_f = false);
 hipError_t launch_skin_arm(komd_ctx* c, int par);  // rebuild word of parity par = 1
 hipError_t launch_kick_drift(komd_ctx* c);      // integrate0
 hipError_t launch_kick(komd_ctx* c, bool with_temp);  // integrate1 (+ kinetic partials)
@@ -263,6 +263,8 @@ hipError_t launch_atm(komd_ctx* c, int atm, bool energy, const float* lam, const
                       bool lj = false, int skin_par = -1);
 // dirs: also the cell direction and rank of every entry (d.vld), for launch_skin_refresh
 hipError_t launch_skin_list(komd_ctx* c, int par, bool dirs);
+// LJ from the Verlet lists + Step1 + rebuild decision (runs without a three-body term)
+hipError_t launch_skin_lj(komd_ctx* c, int par, bool add);
 // the in-range neighbour-list rows (d.nl, d.nlc, d.nld) of every site from its Verlet list
 // at the current positions, and the LJ 12-6 force written as the site's force
 hipError_t launch_skin_refresh(komd_ctx* c);
@@ -279,7 +281,9 @@ hipError_t table_upload(komd_ctx* c, int64_t nodes, const double* x, const doubl
                         const double* u, double u_low, double u_high);
 void table_free(komd_ctx* c);
 // k_bonded.hip
-hipError_t launch_bonded(komd_ctx* c, bool energy);
+// intra: also count the in-range same-molecule pairs into the Verlet-list pair counters
+// (per-molecule grouping only; Verlet lists exclude those pairs)
+hipError_t launch_bonded(komd_ctx* c, bool energy, bool intra = false);
 hipError_t launch_limit(komd_ctx* c);
 // k_surrogate.hip.  Both expect current neighbour lists.  With energy, the three-body
 // energy partials are left in d.red, `three_body_energy_blocks` of them.

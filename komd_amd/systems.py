@@ -209,9 +209,10 @@ def config_lj_4000(seed: int = 1001) -> System:
     return fcc(10, 0.8, 1.0, seed)
 
 
-def config_chains_262144(seed: int = 1002) -> System:
-    """65,536 linear 4-site chains (262,144 sites), site density 0.7."""
-    return chains((64, 32, 32), 4, 0.7, seed=seed)
+def config_chains_262144(seed: int = 1002, list_radius: float = 2.5) -> System:
+    """65,536 linear 4-site chains (262,144 sites), site density 0.7; cells of edge at least
+    list_radius (cutoff 2.5 plus the Verlet-list skin, if any)."""
+    return chains((64, 32, 32), 4, 0.7, seed=seed, rc=list_radius)
 
 
 def config_atm_32000(seed: int = 1003, list_radius: float = 2.5) -> System:

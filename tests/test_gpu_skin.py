@@ -173,8 +173,8 @@ def test_skin_arguments():
 
 
 def test_skin_is_ignored_where_it_does_not_apply():
-    """A thermostatted run and the surrogate-free LJ-only run keep their per-step container
-    update: bit-identical to a context without a skin."""
+    """Thermostatted runs and the reference-mode ATM keep their per-step container update:
+    bit-identical to a context without a skin."""
     s = argon(10, 68)
     with K.Context(s, nu=NU) as a, K.Context(s, nu=NU) as b:
         b.set_skin(SKIN)
@@ -182,11 +182,29 @@ def test_skin_is_ignored_where_it_does_not_apply():
         for c in (a, b):
             c.compute_forces(K.ATM_EXACT)
             c.run(20, kind3b=K.ATM_EXACT, t0=0.9)
-            c.run(20, kind3b=0)
+            c.run(20, kind3b=0, t0=0.9)
+            c.run(10, kind3b=K.ATM_REF)
             res.append(c.get_state())
         assert b.skin_rebuilds() == 0
     for x, y in zip(res[0], res[1]):
         assert np.array_equal(x, y)
+
+
+def test_skin_is_ignored_for_chains_it_cannot_serve():
+    """Multi-site molecules binned by their centre of mass, or run without their bonded
+    terms, keep the per-step container update (bit-identical)."""
+    t, topo = warmed_chains((6, 6, 6), 71)
+    for mode, bonded in ((K.BIN_COM, True), (K.BIN_SITE, False)):
+        out = []
+        for skin in (0.0, SKIN):
+            with K.Context(t, topology=topo, bin_mode=mode) as c:
+                c.set_skin(skin)
+                c.compute_forces(0, bonded=bonded)
+                c.run(20, kind3b=0, bonded=bonded)
+                out.append(c.get_state())
+                assert c.skin_rebuilds() == 0
+        for x, y in zip(out[0], out[1]):
+            assert np.array_equal(x, y)
 
 
 def trained_params():
@@ -227,3 +245,137 @@ def test_skin_learned_run_matches_fresh_forces_and_plain_run(kind, steps):
     assert 1 <= reb < steps // 2
     assert mx < 1e-4 and rms < 1e-5
     assert dr < 1e-5 and dv < 1e-4
+
+
+# ---- runs without a three-body term: LJ (and the bonded terms) from the Verlet lists
+
+def test_skin_lj_only_run(orc):
+    """Single-site LJ without a three-body term: the forces after the run equal the oracle's
+    LJ at the final positions, and every step has the pair set of a per-step run."""
+    s = argon(10, 72)
+    print(f"skin LJ-only n={s.n}", flush=True)
+    with K.Context(s) as a, K.Context(s) as b:
+        b.set_skin(SKIN)
+        out = []
+        for c in (a, b):
+            c.compute_forces(0)
+            c.counters(reset=True)
+            c.run(80, kind3b=0)
+            out.append((c.get_state(), c.counters()))
+        reb = b.skin_rebuilds()
+    (ra, va, _), ca = out[0]
+    (rb, vb, fb), cb = out[1]
+    t = s.copy()
+    t.r = rb
+    rr, bb = orc.update_container(t)
+    mx, rms = rel_err(fb, orc.lj_c08(t, rr, bb, nthreads=8))
+    dr = float(np.abs(min_image(ra - rb, s.box)).max())
+    print(f"  rebuilds {reb}; forces vs oracle max {mx:.2e} rms {rms:.2e}; vs per-step |dr| {dr:.2e}; "
+          f"counts {ca} {cb}")
+    assert ca == cb
+    assert 1 <= reb < 20
+    assert mx < 2e-4 and rms < 5e-6
+    assert dr < 1e-5
+
+
+def warmed_chains(nmol_xyz, seed):
+    """Chains on a site grid of cells >= cutoff + skin after the capped warm-up of the
+    benchmark (force cap raised from 20 to 2e5 in five 300-step stages at dt 0.001 with
+    rescaling to T* 1.0), sites binned on their own positions."""
+    s = S.chains(nmol_xyz, 4, 0.7, seed=seed, rc=2.5 + SKIN)
+    topo = {"bonds": s.bonds, "angles": s.angles}
+    for lim in (20.0, 200.0, 2000.0, 2e4, 2e5):
+        with K.Context(s, cutoff=2.5, dt=0.001, limit_factor=lim, topology=topo, bin_mode=K.BIN_SITE) as c:
+            c.compute_forces(0, bonded=True)
+            c.run(300, kind3b=0, bonded=True, t0=1.0)
+            s.r, s.v, _ = c.get_state()
+    return s, topo
+
+
+def chain_oracle_forces(orc, s, r, nthreads=8):
+    """FP64 LJ between molecules over the site grid (the chains' ids exclude same-molecule
+    pairs, LJ12_6.cpp:19) plus the harmonic bonds and angles; returns (wrapped r, f)."""
+    single = s.copy()
+    single.mol_id = np.arange(s.n, dtype=np.uint64)
+    rr, b = orc.update_container(single, r)
+    return rr, orc.lj_c08(s, rr, b, nthreads=nthreads) + orc.bonded(s, rr)[0]
+
+
+def chain_runs(s, topo, steps):
+    """The same NVE run per step and on Verlet lists: (state, counters) of each, rebuilds."""
+    with K.Context(s, topology=topo, bin_mode=K.BIN_SITE) as a, \
+            K.Context(s, topology=topo, bin_mode=K.BIN_SITE) as b:
+        b.set_skin(SKIN)
+        out = []
+        for c in (a, b):
+            c.compute_forces(0, bonded=True)
+            c.counters(reset=True)
+            c.run(steps, kind3b=0, bonded=True)
+            out.append((c.get_state(), c.counters()))
+        return out, b.skin_rebuilds()
+
+
+def test_skin_chains_follow_fp64_run(orc):
+    """2,048 sites of bonded four-site chains: the same pair set every step as the per-step
+    run (the lists leave same-molecule pairs out; the bonded kernel counts the in-range ones),
+    forces at the end equal to the oracle's, and both GPU runs track the FP64 trajectory of
+    the oracle (bonded + LJ between molecules, reference step order) equally well."""
+    s, topo = warmed_chains((8, 8, 8), 73)
+    steps = 60
+    print(f"skin chains n={s.n} steps={steps}", flush=True)
+    out, reb = chain_runs(s, topo, steps)
+    (ra, va, _), ca = out[0]
+    (rb, vb, fb), cb = out[1]
+    _, f_or = chain_oracle_forces(orc, s, rb)
+    mx, rms = rel_err(fb, f_or)
+    dt = 0.005
+    r, f = chain_oracle_forces(orc, s, s.r)
+    v = s.v.copy()
+    for _ in range(steps):
+        r, v = orc.vv_step0(dt, r, v, f, s.mass)
+        r, f = chain_oracle_forces(orc, s, r)
+        v = orc.vv_step1(dt, v, f, s.mass)
+    ea = float(np.abs(min_image(ra - r, s.box)).max())
+    eb = float(np.abs(min_image(rb - r, s.box)).max())
+    eva, evb = float(np.abs(va - v).max()), float(np.abs(vb - v).max())
+    print(f"  rebuilds {reb}; forces vs oracle max {mx:.2e} rms {rms:.2e}; vs FP64: per-step |dr| {ea:.2e} "
+          f"|dv| {eva:.2e}, skinned |dr| {eb:.2e} |dv| {evb:.2e}; counts {ca} {cb}")
+    assert ca == cb, "the same in-range pairs every step"
+    assert 1 <= reb < steps // 3
+    assert mx < 2e-4 and rms < 1e-5
+    assert eb < 3.0 * ea + 1e-9 and evb < 3.0 * eva + 1e-8
+    assert eb < 1e-5 and evb < 1e-4
+
+
+def test_skin_chains_262144_full_size(orc):
+    """The chain workload at its full size (65,536 chains) on Verlet lists after the
+    benchmark's warm-up, 40 steps: forces at the end equal to the oracle's, and the run
+    tracks the FP64 trajectory as closely as the per-step run does.  The two GPU runs part at
+    the FP32 level and the chains amplify that (|dr| ~2e-5 after 100 steps), so a pair that
+    close to the cutoff may be counted on either side: the counts agree to 1e-8 here, exactly
+    in the 2,048-site test."""
+    s, topo = warmed_chains((64, 32, 32), 1002)
+    steps = 40
+    print(f"skin chains full size n={s.n} steps={steps}", flush=True)
+    out, reb = chain_runs(s, topo, steps)
+    (ra, va, _), ca = out[0]
+    (rb, vb, fb), cb = out[1]
+    _, f_or = chain_oracle_forces(orc, s, rb, nthreads=16)
+    mx, rms = rel_err(fb, f_or)
+    dt = 0.005
+    r, f = chain_oracle_forces(orc, s, s.r, nthreads=16)
+    v = s.v.copy()
+    for _ in range(steps):
+        r, v = orc.vv_step0(dt, r, v, f, s.mass)
+        r, f = chain_oracle_forces(orc, s, r, nthreads=16)
+        v = orc.vv_step1(dt, v, f, s.mass)
+    ea = float(np.abs(min_image(ra - r, s.box)).max())
+    eb = float(np.abs(min_image(rb - r, s.box)).max())
+    eva, evb = float(np.abs(va - v).max()), float(np.abs(vb - v).max())
+    print(f"  rebuilds {reb}; forces vs oracle max {mx:.2e} rms {rms:.2e}; vs FP64: per-step |dr| {ea:.2e} "
+          f"|dv| {eva:.2e}, skinned |dr| {eb:.2e} |dv| {evb:.2e}; counts {ca} {cb}")
+    assert abs(ca[0] - cb[0]) <= 1e-8 * ca[0] and ca[1] == cb[1] == 0
+    assert 1 <= reb < steps // 3
+    assert mx < 2e-4 and rms < 1e-5
+    assert eb < 3.0 * ea + 1e-9 and evb < 3.0 * eva + 1e-8
+    assert eb < 1e-4 and evb < 1e-3
