@@ -125,7 +125,9 @@ komd_status komd_grid_dims(const komd_ctx* ctx, int64_t dims[3]);
  * Consecutive sites with equal ids form one molecule (CheckpointIO.cpp:63-72).
  * Replaces MoleculeContainer::constructSOAs (MoleculeContainer.cpp:37-53).
  * Forces are zeroed.  A call with a different site count than the last one drops the
- * surrogate training set (its snapshots hold the old count); surrogate parameters stay. */
+ * surrogate training set (its snapshots hold the old count); surrogate parameters stay.
+ * A call that changes the site count or the molecules drops the bonded topology
+ * (komd_set_topology indexes sites and groups its terms by molecule). */
 komd_status komd_set_sites(komd_ctx* ctx, int64_t n, const double* r, const double* v, const double* mass,
                            const double* eps, const double* sig, const uint64_t* mol_id);
 /* Overwrites positions / velocities / forces (any pointer may be NULL). */
@@ -238,9 +240,9 @@ komd_status komd_compute_forces(komd_ctx* ctx, int32_t kind3b, int32_t use_bonde
  * interaction set therefore equals the one of a container update at that step.  Applies to
  * komd_run without thermostat (t0 <= 0) with LJ 12-6 of uniform sites without the clamp and
  * either KOMD_3B_ATM_EXACT, KOMD_3B_SURROGATE or KOMD_3B_HYBRID (single-site, no bonded
- * terms), or no three-body term: single-site molecules (bonded terms allowed), or multi-site
- * molecules with KOMD_BIN_SITE and bonded terms inside molecules, use_bonded set (their lists
- * leave same-molecule pairs out, LJ12_6.cpp:19); other runs ignore it.  The run's last
+ * terms), or no three-body term: single-site molecules, or multi-site molecules with
+ * KOMD_BIN_SITE whose ids each form one run of consecutive sites (their lists leave
+ * same-molecule pairs out, LJ12_6.cpp:19); bonded terms allowed; other runs ignore it.  The run's last
  * container update wraps the positions and leaves the grid consistent with them.
  * skin = 0 (default) turns it off; skin > 0 needs cell_size >= cutoff + skin. */
 komd_status komd_set_skin(komd_ctx* ctx, double skin);

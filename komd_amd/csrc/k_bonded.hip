@@ -245,6 +245,119 @@ __global__ __launch_bounds__(256) void k_bonded_mol(int nmol, const uint32_t* __
     }
 }
 
+// Bonded forces per site (force passes of topologies whose terms cross molecules, in place of
+// the atomics of k_bonds / k_angles): each site sums the bond and angle terms it takes part
+// in, with their FP32 expressions per term, in FP64 in the order of its term list, and adds
+// the total to its force once (a fixed order).  Term reference: term << 3 | angle << 2 | role (bond: 0 first site, 1 second;
+// angle: 0, 1 middle, 2).  intra (Verlet-list runs of multi-site molecules, whose lists leave
+// same-molecule pairs out): also count the in-range pairs (i, j > i) of the site's molecule,
+// decided as the pair pass decides them, into the Verlet-list pair counters (without terms:
+// the count alone, for runs without bonded forces).
+constexpr int kSiteTerms = 6;
+constexpr uint32_t kNoTerm = 0xffffffffu;
+
+__device__ __forceinline__ void site_term(uint32_t e, const uint32_t* __restrict__ bonds,
+                                          const uint32_t* __restrict__ ang, float kb, float r0, float ka, float th0,
+                                          const double* __restrict__ rx, const double* __restrict__ ry,
+                                          const double* __restrict__ rz, const Grid& g, double& sx, double& sy,
+                                          double& sz) {
+    const uint32_t t = e >> 3, role = e & 3u;
+    if (e & 4u) {
+        const uint32_t i = ang[3 * t], j = ang[3 * t + 1], k = ang[3 * t + 2];
+        const float ax = min_image(rx[i] - rx[j], g.size[0]);
+        const float ay = min_image(ry[i] - ry[j], g.size[1]);
+        const float az = min_image(rz[i] - rz[j], g.size[2]);
+        const float bx = min_image(rx[k] - rx[j], g.size[0]);
+        const float by = min_image(ry[k] - ry[j], g.size[1]);
+        const float bz = min_image(rz[k] - rz[j], g.size[2]);
+        const float la2 = ax * ax + ay * ay + az * az, lb2 = bx * bx + by * by + bz * bz;
+        const float inv_ab = rsqrtf(la2 * lb2);
+        float c = (ax * bx + ay * by + az * bz) * inv_ab;
+        c = fminf(1.f, fmaxf(-1.f, c));
+        const float th = acosf(c);
+        const float dth = th - th0;
+        const float s = fmaxf(sqrtf(fmaxf(0.f, 1.f - c * c)), 1e-6f);
+        const float pre = ka * dth / s;
+        const float ia2 = 1.f / la2, ib2 = 1.f / lb2;
+        const float fix = pre * (bx * inv_ab - c * ax * ia2), fiy = pre * (by * inv_ab - c * ay * ia2),
+                    fiz = pre * (bz * inv_ab - c * az * ia2);
+        const float fkx = pre * (ax * inv_ab - c * bx * ib2), fky = pre * (ay * inv_ab - c * by * ib2),
+                    fkz = pre * (az * inv_ab - c * bz * ib2);
+        if (role == 0) {
+            sx += (double)fix; sy += (double)fiy; sz += (double)fiz;
+        } else if (role == 2) {
+            sx += (double)fkx; sy += (double)fky; sz += (double)fkz;
+        } else {
+            sx -= (double)(fix + fkx); sy -= (double)(fiy + fky); sz -= (double)(fiz + fkz);
+        }
+    } else {
+        const uint32_t i = bonds[2 * t], j = bonds[2 * t + 1];
+        const float dx = min_image(rx[j] - rx[i], g.size[0]);
+        const float dy = min_image(ry[j] - ry[i], g.size[1]);
+        const float dz = min_image(rz[j] - rz[i], g.size[2]);
+        const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float dl = len - r0;
+        const float gfac = -kb * dl / len;
+        if (role == 1) {
+            sx += (double)(gfac * dx); sy += (double)(gfac * dy); sz += (double)(gfac * dz);
+        } else {
+            sx -= (double)(gfac * dx); sy -= (double)(gfac * dy); sz -= (double)(gfac * dz);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bonded_site(int n, const uint32_t* __restrict__ tstart,
+                                                     const uint32_t* __restrict__ terms,
+                                                     const uint32_t* __restrict__ bonds,
+                                                     const uint32_t* __restrict__ ang, float kb, float r0, float ka,
+                                                     float th0, const double* __restrict__ rx,
+                                                     const double* __restrict__ ry, const double* __restrict__ rz,
+                                                     double* __restrict__ fx, double* __restrict__ fy,
+                                                     double* __restrict__ fz, Grid g,
+                                                     const uint32_t* __restrict__ mol_of,
+                                                     const uint32_t* __restrict__ mol_start, double rc2,
+                                                     unsigned long long* __restrict__ intra, int nslots) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nintra = 0;
+    if (i < n) {
+        if (tstart) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const uint32_t t0 = tstart[i], t1 = tstart[i + 1];
+            for (uint32_t tb = t0; tb < t1; tb += kSiteTerms) {
+                uint32_t e[kSiteTerms];
+#pragma unroll
+                for (int k = 0; k < kSiteTerms; k++) e[k] = tb + k < t1 ? terms[tb + k] : kNoTerm;
+#pragma unroll
+                for (int k = 0; k < kSiteTerms; k++)
+                    if (e[k] != kNoTerm) site_term(e[k], bonds, ang, kb, r0, ka, th0, rx, ry, rz, g, sx, sy, sz);
+            }
+            fx[i] = fx[i] + sx;
+            fy[i] = fy[i] + sy;
+            fz[i] = fz[i] + sz;
+        }
+        if (intra) {
+            const uint32_t s1 = mol_start[mol_of[i] + 1];
+            const double xi = rx[i], yi = ry[i], zi = rz[i];
+            for (uint32_t jb = (uint32_t)i + 1; jb < s1; jb += kMolRegs) {
+                double x[kMolRegs], y[kMolRegs], z[kMolRegs];
+#pragma unroll
+                for (int u = 0; u < kMolRegs; u++) {
+                    const uint32_t j = jb + u < s1 ? jb + u : (uint32_t)i;
+                    x[u] = rx[j]; y[u] = ry[j]; z[u] = rz[j];
+                }
+#pragma unroll
+                for (int u = 0; u < kMolRegs; u++)
+                    if (jb + u < s1) nintra += intra_in_range(xi, yi, zi, x[u], y[u], z[u], g, rc2);
+            }
+        }
+    }
+    if (intra) {
+        // one add per wave, each workgroup to its own slot (same-address atomics serialise)
+        for (int o = 32; o > 0; o >>= 1) nintra += __shfl_xor(nintra, o);
+        if ((threadIdx.x & 63) == 0 && nintra) atomicAdd(&intra[blockIdx.x % nslots], (unsigned long long)nintra);
+    }
+}
+
 // Limit_Force (Limit.cpp:19-25): f = max(min(f, fmax), -fmax), fmax = limit eps / sig.
 __global__ __launch_bounds__(256) void k_limit(int n, double lim, const double* __restrict__ eps,
                                                const double* __restrict__ sig, double* __restrict__ fx,
@@ -267,7 +380,8 @@ hipError_t launch_bonded(komd_ctx* c, bool energy, bool intra) {
     ScopedTimer tm(c, "bonded");
     auto& d = c->d;
     const int blocks = 256;
-    if (intra && (!c->bonded_by_mol || !d.mol_start || !d.cnt_trip)) return hipErrorInvalidValue;
+    if (intra && (!d.mol_start || !d.cnt_trip || !c->ids_runs)) return hipErrorInvalidValue;
+    if (!energy && !c->bonded_by_mol) return launch_bonded_sites(c, true, intra);
     if (c->bonded_by_mol) {
         const int nmol = (int)c->nmol;
         // the Verlet-list passes' first pair-count slot
@@ -292,6 +406,21 @@ hipError_t launch_bonded(komd_ctx* c, bool energy, bool intra) {
         KOMD_CHECK_HIP(hipGetLastError());
     }
     return hipSuccess;
+}
+
+hipError_t launch_bonded_sites(komd_ctx* c, bool terms, bool intra) {
+    ScopedTimer tm(c, "bonded");
+    auto& d = c->d;
+    if ((terms && (!d.site_tstart || !d.site_terms)) ||
+        (intra && (!d.mol_start || !d.idcls || !d.cnt_trip || !c->ids_runs)))
+        return hipErrorInvalidValue;
+    if (!terms && !intra) return hipSuccess;
+    const int n = (int)c->n;
+    k_bonded_site<<<(n + 255) / 256, 256, 0, c->stream>>>(
+        n, terms ? d.site_tstart : nullptr, d.site_terms, d.bonds, d.angles, (float)c->kb, (float)c->r0,
+        (float)c->ka, (float)c->theta0, d.rx, d.ry, d.rz, d.fx, d.fy, d.fz, c->grid, d.idcls, d.mol_start,
+        c->prm.cutoff * c->prm.cutoff, intra ? d.cnt_trip + atm_blocks(c) : nullptr, atm_blocks(c));
+    return hipGetLastError();
 }
 
 hipError_t launch_limit(komd_ctx* c) {

@@ -489,6 +489,7 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
             int cnt = 0;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
+                if (h == 1 && nv <= 64) break;  // wave-uniform: a list of at most 64 entries
                 const int s = h * 64 + lane;
                 const bool live = s < nv;
                 const uint32_t e = live ? (h ? e1 : e0) : (uint32_t)p;

@@ -78,9 +78,22 @@ void free_sites(komd_ctx* c) {
     keep.bond_mol = d.bond_mol;
     keep.angle_mol = d.angle_mol;
     keep.angles = d.angles;
+    keep.site_tstart = d.site_tstart;
+    keep.site_terms = d.site_terms;
     d = keep;
     c->have_sites = false;
     c->nl_valid = false;
+}
+
+// Frees the bonded topology (terms, per-molecule and per-site tables): no bonded terms.
+void drop_topology(komd_ctx* c) {
+    auto& d = c->d;
+    void* old[] = {d.bonds, d.angles, d.bond_mol, d.angle_mol, d.site_tstart, d.site_terms};
+    for (void* p : old)
+        if (p) hipFree(p);
+    d.bonds = d.angles = d.bond_mol = d.angle_mol = d.site_tstart = d.site_terms = nullptr;
+    c->nbonds = c->nangles = 0;
+    c->bonded_by_mol = false;
 }
 
 void drop_graph(komd_ctx* c) {
@@ -286,13 +299,12 @@ hipError_t enqueue_forces(komd_ctx* c, int kind3b, bool bonded) {
 
 // Runs that use Verlet lists (komd_set_skin): NVE with LJ 12-6 of uniform sites without the
 // clamp, and either the exact ATM, the surrogate or the hybrid slab (single-site, no bonded
-// terms), or no three-body term: single-site sites (bonded terms allowed), or multi-site
-// molecules binned by site whose bonded terms are grouped by molecule (that kernel counts
-// the same-molecule pairs the lists leave out).
+// terms), or no three-body term: single-site sites, or multi-site molecules binned by site
+// whose ids each form one run of sites (the per-site bonded kernel counts the same-molecule
+// pairs the lists leave out, from the molecule ranges).
 bool skin_run(const komd_ctx* c, int kind3b, bool bonded, bool thermo) {
     const bool three = kind3b == KOMD_3B_ATM_EXACT || kind3b == KOMD_3B_SURROGATE || kind3b == KOMD_3B_HYBRID;
-    const bool two = kind3b == 0 &&
-                     (!c->multisite || (c->prm.bin_mode == KOMD_BIN_SITE && bonded && c->bonded_by_mol));
+    const bool two = kind3b == 0 && (!c->multisite || (c->prm.bin_mode == KOMD_BIN_SITE && c->ids_runs));
     const bool sites = three ? (!bonded && !c->multisite) : two;
     return c->skin > 0 && c->n <= (int64_t)kSkinIdxMask && sites && !thermo &&
            c->pair_kind == KOMD_PAIR_LJ12_6 && c->uniform && !(c->prm.limit_factor > 0);
@@ -308,13 +320,17 @@ bool skin_run(const komd_ctx* c, int kind3b, bool bonded, bool thermo) {
 // lists added to them with Step1 fused.
 hipError_t enqueue_skin_step(komd_ctx* c, int kind3b, int par, bool bonded) {
     const bool two = kind3b == 0;
-    KOMD_CHECK_HIP(launch_skin_drift(c, par, two && bonded));
+    // multi-site molecules: a bonded kernel runs (if only to count the in-range same-molecule
+    // pairs) and the forces add onto the zero the drift writes
+    const bool sites = two && (bonded || c->multisite);
+    KOMD_CHECK_HIP(launch_skin_drift(c, par, sites));
     KOMD_CHECK_HIP(launch_bin(c, c->d.skin_req + par, c->d.spb));
     const bool exact = kind3b == KOMD_3B_ATM_EXACT;
     KOMD_CHECK_HIP(launch_skin_list(c, par, !exact && !two));
     if (two) {
         if (bonded) KOMD_CHECK_HIP(launch_bonded(c, false, c->multisite));
-        return launch_skin_lj(c, par, bonded);
+        else if (sites) KOMD_CHECK_HIP(launch_bonded_sites(c, false, true));  // the count alone
+        return launch_skin_lj(c, par, sites);
     }
     hipError_t e;
     if (exact) {
@@ -434,10 +450,7 @@ void komd_destroy(komd_ctx* c) {
     if (c->flag_host) hipHostFree(c->flag_host);
     for (auto e : c->flag_ev)
         if (e) hipEventDestroy(e);
-    if (c->d.bonds) hipFree(c->d.bonds);
-    if (c->d.angles) hipFree(c->d.angles);
-    if (c->d.bond_mol) hipFree(c->d.bond_mol);
-    if (c->d.angle_mol) hipFree(c->d.angle_mol);
+    drop_topology(c);
     for (auto& pe : c->pending) {
         hipEventDestroy(pe.second.first);
         hipEventDestroy(pe.second.second);
@@ -463,6 +476,7 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     if (n > (int64_t)kIdxMask) return fail(c, KOMD_ERR_ARG, "at most 2^26-1 sites");
     drop_graph(c);
     free_sites(c);
+    const int64_t n_old = c->n;
     c->n = n;
     surrogate_sites_changed(c);
     auto& d = c->d;
@@ -472,6 +486,9 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
     for (int64_t i = 1; i < n; i++)
         if (mol_id[i] != mol_id[i - 1]) mstart.push_back((uint32_t)i);
     mstart.push_back((uint32_t)n);
+    // the bonded terms index sites and are grouped by molecule: they hold only for the same
+    // sites and molecules
+    if (n != n_old || mstart != c->mol_start_h) drop_topology(c);
     c->nmol = (int64_t)mstart.size() - 1;
     c->mol_start_h = mstart;
     c->multisite = c->nmol != n;
@@ -483,6 +500,7 @@ komd_status komd_set_sites(komd_ctx* c, int64_t n, const double* r, const double
         idc[(size_t)i] = it->second;
     }
     if (!c->multisite && (int64_t)cls.size() != n) c->multisite = true;  // repeated ids: exclusion matters
+    c->ids_runs = (int64_t)cls.size() == c->nmol;  // no id reappears after another
     c->uniform = true;
     for (int64_t i = 1; i < n; i++)
         if (eps[i] != eps[0] || sig[i] != sig[0]) { c->uniform = false; break; }
@@ -609,11 +627,9 @@ komd_status komd_set_topology(komd_ctx* c, int64_t nb, const uint32_t* bonds, do
         if ((int64_t)bonds[i] >= c->n) return fail(c, KOMD_ERR_ARG, "bond index out of range");
     for (int64_t i = 0; i < 3 * na; i++)
         if ((int64_t)angles[i] >= c->n) return fail(c, KOMD_ERR_ARG, "angle index out of range");
+    if (nb >= (int64_t)1 << 29 || na >= (int64_t)1 << 29) return fail(c, KOMD_ERR_ARG, "at most 2^29-1 bonds and angles");
     drop_graph(c);
-    void* old[] = {c->d.bonds, c->d.angles, c->d.bond_mol, c->d.angle_mol};
-    for (void* p : old)
-        if (p) hipFree(p);
-    c->d.bonds = c->d.angles = c->d.bond_mol = c->d.angle_mol = nullptr;
+    drop_topology(c);
     // Group the terms by molecule when every bond and angle lies inside one molecule: one
     // thread per molecule then updates its own sites without atomics.
     const std::vector<uint32_t>& ms = c->mol_start_h;
@@ -651,6 +667,25 @@ komd_status komd_set_topology(komd_ctx* c, int64_t nb, const uint32_t* bonds, do
         HIPCALL(c, h2d(c, c->d.angle_mol, a_start.data(), a_start.size() * 4), "komd_set_topology");
     }
     c->bonded_by_mol = by_mol;
+    {
+        // per site: the terms it takes part in (term << 3 | angle << 2 | role), bonds first,
+        // each in the order of the uploaded term arrays
+        const size_t n = (size_t)c->n;
+        std::vector<uint32_t> ts(n + 1, 0), tl((size_t)(2 * nb + 3 * na));
+        for (int64_t t = 0; t < 2 * nb; t++) ts[b_sorted[(size_t)t] + 1]++;
+        for (int64_t t = 0; t < 3 * na; t++) ts[a_sorted[(size_t)t] + 1]++;
+        for (size_t i = 0; i < n; i++) ts[i + 1] += ts[i];
+        std::vector<uint32_t> pos(ts.begin(), ts.end() - 1);
+        for (int64_t t = 0; t < nb; t++)
+            for (uint32_t r = 0; r < 2; r++) tl[pos[b_sorted[(size_t)(2 * t + r)]]++] = ((uint32_t)t << 3) | r;
+        for (int64_t t = 0; t < na; t++)
+            for (uint32_t r = 0; r < 3; r++)
+                tl[pos[a_sorted[(size_t)(3 * t + r)]]++] = ((uint32_t)t << 3) | 4u | r;
+        HIPCALL(c, dalloc(&c->d.site_tstart, n + 1), "komd_set_topology");
+        HIPCALL(c, dalloc(&c->d.site_terms, std::max<size_t>(1, tl.size())), "komd_set_topology");
+        HIPCALL(c, h2d(c, c->d.site_tstart, ts.data(), (n + 1) * 4), "komd_set_topology");
+        if (!tl.empty()) HIPCALL(c, h2d(c, c->d.site_terms, tl.data(), tl.size() * 4), "komd_set_topology");
+    }
     HIPCALL(c, dalloc(&c->d.bonds, (size_t)std::max<int64_t>(1, 2 * nb)), "komd_set_topology");
     HIPCALL(c, dalloc(&c->d.angles, (size_t)std::max<int64_t>(1, 3 * na)), "komd_set_topology");
     if (nb) HIPCALL(c, h2d(c, c->d.bonds, b_sorted.data(), (size_t)(2 * nb) * 4), "komd_set_topology");

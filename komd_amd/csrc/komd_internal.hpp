@@ -94,6 +94,8 @@ struct DevState {
     // bonded
     uint32_t *bonds = nullptr, *angles = nullptr;
     uint32_t *bond_mol = nullptr, *angle_mol = nullptr;  // CSR of bonds / angles per molecule
+    // per site: the bond and angle terms it takes part in (CSR: term << 3 | angle << 2 | role)
+    uint32_t *site_tstart = nullptr, *site_terms = nullptr;
     // reductions and flags
     double* red = nullptr;        // partials (see k_container.hip)
     double* vpart = nullptr;      // [6][blocks] virial / kinetic tensor partials (energy passes)
@@ -131,6 +133,7 @@ struct komd_ctx {
     int64_t nbonds = 0, nangles = 0;
     std::vector<uint32_t> mol_start_h;  // host copy of the molecule ranges
     bool bonded_by_mol = false;         // every bond/angle inside one molecule: per-molecule kernel
+    bool ids_runs = true;               // every molecule id forms one run of consecutive sites
     double kb = 0, r0 = 0, ka = 0, theta0 = 0;
     hipStream_t stream = nullptr;
     komd::DevState d;
@@ -283,7 +286,11 @@ void table_free(komd_ctx* c);
 // k_bonded.hip
 // intra: also count the in-range same-molecule pairs into the Verlet-list pair counters
 // (per-molecule grouping only; Verlet lists exclude those pairs)
+// intra: also count the in-range same-molecule pairs into the Verlet-list pair counters
+// (Verlet lists exclude those pairs)
 hipError_t launch_bonded(komd_ctx* c, bool energy, bool intra = false);
+// the per-site bonded kernel (terms: add the bond and angle forces; intra: the count)
+hipError_t launch_bonded_sites(komd_ctx* c, bool terms, bool intra);
 hipError_t launch_limit(komd_ctx* c);
 // k_surrogate.hip.  Both expect current neighbour lists.  With energy, the three-body
 // energy partials are left in d.red, `three_body_energy_blocks` of them.

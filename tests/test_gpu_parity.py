@@ -320,6 +320,30 @@ def test_bonded_forces(orc):
     assert abs(obs["u_bond"] + obs["u_angle"] - u_or) < 1e-4 * abs(u_or)
 
 
+def test_bonded_forces_across_molecules(orc):
+    """Bonds and angles between sites of different molecules (every site its own molecule):
+    the force pass sums each site's terms itself, in a fixed order, with the arithmetic of the
+    per-molecule kernel; the energies come from the per-term kernels."""
+    s = S.chains((6, 6, 6), 4, 0.7, seed=10)
+    s.r = s.r + np.random.default_rng(10).normal(size=s.r.shape) * 0.05
+    s.mol_id = np.arange(s.n, dtype=np.uint64)
+    r, _ = orc.update_container(s)
+    f_or, u_or = orc.bonded(s, r)
+    with K.Context(s, topology={"bonds": s.bonds, "angles": s.angles}) as c:
+        c.update_container()
+        c.forces_bonded()
+        f = c.forces()
+        c.update_container()
+        c.forces_bonded()
+        f2 = c.forces()
+        obs = c.observables()
+    mx, rms = rel_err(f, f_or)
+    print(f"bonded across molecules max {mx:.3e} rms {rms:.3e}")
+    assert np.array_equal(f, f2), "a fixed summation order"
+    assert mx < 1e-4 and rms < 1e-5
+    assert abs(obs["u_bond"] + obs["u_angle"] - u_or) < 1e-4 * abs(u_or)
+
+
 def test_limit(orc):
     s = small(61, 5, jitter=0.15)
     r, b = orc.update_container(s)

@@ -191,60 +191,46 @@ def test_skin_is_ignored_where_it_does_not_apply():
 
 
 def test_skin_is_ignored_for_chains_it_cannot_serve():
-    """Multi-site molecules binned by their centre of mass, or run without their bonded
-    terms, keep the per-step container update (bit-identical)."""
+    """Multi-site molecules binned by their centre of mass, or whose molecule id reappears
+    after another molecule (the same-molecule exclusion then spans several runs of sites),
+    keep the per-step container update (bit-identical)."""
     t, topo = warmed_chains((6, 6, 6), 71)
-    for mode, bonded in ((K.BIN_COM, True), (K.BIN_SITE, False)):
+    u = t.copy()
+    u.mol_id = t.mol_id.copy()
+    u.mol_id[-4:] = t.mol_id[0]  # the last chain takes the first chain's id
+    for sysm, mode in ((t, K.BIN_COM), (u, K.BIN_SITE)):
         out = []
         for skin in (0.0, SKIN):
-            with K.Context(t, topology=topo, bin_mode=mode) as c:
+            with K.Context(sysm, topology=topo, bin_mode=mode) as c:
                 c.set_skin(skin)
-                c.compute_forces(0, bonded=bonded)
-                c.run(20, kind3b=0, bonded=bonded)
+                c.compute_forces(0, bonded=True)
+                c.run(20, kind3b=0, bonded=True)
                 out.append(c.get_state())
                 assert c.skin_rebuilds() == 0
         for x, y in zip(out[0], out[1]):
             assert np.array_equal(x, y)
 
 
-def trained_params():
-    from komd_amd.train import DEFAULT_WEIGHTS
-    return np.load(DEFAULT_WEIGHTS)
-
-
-@pytest.mark.parametrize("kind,steps", [(K.SURROGATE, 60), (K.HYBRID, 40)])
-def test_skin_learned_run_matches_fresh_forces_and_plain_run(kind, steps):
-    """The learned and hybrid kinds on Verlet lists: each step's rows come from the lists at
-    the current positions (k_skin_refresh), so the forces at the end of the run equal a fresh
-    force pass (container update, lists, LJ, three-body) at the same positions to the FP32
-    rounding of the list vectors, and the run follows the per-step run to the FP32 level of
-    the relative coordinates (the surrogate sums its descriptors in the list order)."""
-    s = argon(10, 70)
-    p = trained_params()
-    center = float(s.low[0] + 0.5 * s.box[0])
-    print(f"skin learned kind={kind} n={s.n} steps={steps}", flush=True)
-    with K.Context(s, nu=NU) as a, K.Context(s, nu=NU) as b:
-        b.set_skin(SKIN)
-        out = []
-        for c in (a, b):
-            c.surrogate_set_params(p)
-            if kind == K.HYBRID:
-                c.set_hybrid(0, center, 2.0, 2.0)
-            c.compute_forces(kind)
-            c.run(steps, kind3b=kind)
-            out.append(c.get_state())
-        reb = b.skin_rebuilds()
-        rb, vb, fb = out[1]
-        b.compute_forces(kind)               # a fresh pass at the run's final positions
-        f_fresh = b.forces()
-    ra, va, fa = out[0]
-    mx, rms = rel_err(fb, f_fresh)
-    dr = float(np.abs(min_image(ra - rb, s.box)).max())
-    dv = float(np.abs(va - vb).max())
-    print(f"  rebuilds {reb}; forces vs fresh max {mx:.2e} rms {rms:.2e}; vs per-step run |dr| {dr:.2e} |dv| {dv:.2e}")
-    assert 1 <= reb < steps // 2
-    assert mx < 1e-4 and rms < 1e-5
-    assert dr < 1e-5 and dv < 1e-4
+def test_set_sites_drops_a_topology_it_no_longer_fits():
+    """The bonded terms index sites and are grouped by molecule: new sites with the same count
+    and molecules keep them, any other count drops them (no bonded force, no bonded energy)."""
+    t, topo = warmed_chains((6, 6, 6), 74)
+    with K.Context(t, topology=topo, bin_mode=K.BIN_SITE) as c:
+        c.update_container()
+        c.forces_bonded()
+        f_b = c.forces()
+        c.set_sites(t)                      # same sites and molecules: terms kept
+        c.update_container()
+        c.forces_bonded()
+        assert np.array_equal(c.forces(), f_b) and np.abs(f_b).max() > 0
+        assert c.observables(0)["u_bond"] > 0
+        s2 = S.chains((4, 4, 4), 4, 0.7, seed=75, rc=2.5 + SKIN)
+        c.set_sites(s2)                     # another count: terms dropped
+        c.update_container()
+        c.forces_bonded()
+        assert not c.forces().any()
+        o = c.observables(0)
+        assert o["u_bond"] == 0 and o["u_angle"] == 0
 
 
 # ---- runs without a three-body term: LJ (and the bonded terms) from the Verlet lists
@@ -292,48 +278,50 @@ def warmed_chains(nmol_xyz, seed):
     return s, topo
 
 
-def chain_oracle_forces(orc, s, r, nthreads=8):
+def chain_oracle_forces(orc, s, r, nthreads=8, bonded=True):
     """FP64 LJ between molecules over the site grid (the chains' ids exclude same-molecule
     pairs, LJ12_6.cpp:19) plus the harmonic bonds and angles; returns (wrapped r, f)."""
     single = s.copy()
     single.mol_id = np.arange(s.n, dtype=np.uint64)
     rr, b = orc.update_container(single, r)
-    return rr, orc.lj_c08(s, rr, b, nthreads=nthreads) + orc.bonded(s, rr)[0]
+    f = orc.lj_c08(s, rr, b, nthreads=nthreads)
+    return rr, (f + orc.bonded(s, rr)[0]) if bonded else f
 
 
-def chain_runs(s, topo, steps):
+def chain_runs(s, topo, steps, bonded=True):
     """The same NVE run per step and on Verlet lists: (state, counters) of each, rebuilds."""
     with K.Context(s, topology=topo, bin_mode=K.BIN_SITE) as a, \
             K.Context(s, topology=topo, bin_mode=K.BIN_SITE) as b:
         b.set_skin(SKIN)
         out = []
         for c in (a, b):
-            c.compute_forces(0, bonded=True)
+            c.compute_forces(0, bonded=bonded)
             c.counters(reset=True)
-            c.run(steps, kind3b=0, bonded=True)
+            c.run(steps, kind3b=0, bonded=bonded)
             out.append((c.get_state(), c.counters()))
         return out, b.skin_rebuilds()
 
 
-def test_skin_chains_follow_fp64_run(orc):
-    """2,048 sites of bonded four-site chains: the same pair set every step as the per-step
-    run (the lists leave same-molecule pairs out; the bonded kernel counts the in-range ones),
-    forces at the end equal to the oracle's, and both GPU runs track the FP64 trajectory of
-    the oracle (bonded + LJ between molecules, reference step order) equally well."""
+@pytest.mark.parametrize("bonded,steps", [(True, 60), (False, 30)])
+def test_skin_chains_follow_fp64_run(orc, bonded, steps):
+    """2,048 sites of four-site chains, with their bonds and angles or without (LJ between
+    molecules only): the same pair set every step as the per-step run (the lists leave
+    same-molecule pairs out; the per-site bonded kernel counts the in-range ones), forces at
+    the end equal to the oracle's, and both GPU runs track the FP64 trajectory of the oracle
+    (reference step order) equally well."""
     s, topo = warmed_chains((8, 8, 8), 73)
-    steps = 60
-    print(f"skin chains n={s.n} steps={steps}", flush=True)
-    out, reb = chain_runs(s, topo, steps)
+    print(f"skin chains n={s.n} bonded={bonded} steps={steps}", flush=True)
+    out, reb = chain_runs(s, topo, steps, bonded)
     (ra, va, _), ca = out[0]
     (rb, vb, fb), cb = out[1]
-    _, f_or = chain_oracle_forces(orc, s, rb)
+    _, f_or = chain_oracle_forces(orc, s, rb, bonded=bonded)
     mx, rms = rel_err(fb, f_or)
     dt = 0.005
-    r, f = chain_oracle_forces(orc, s, s.r)
+    r, f = chain_oracle_forces(orc, s, s.r, bonded=bonded)
     v = s.v.copy()
     for _ in range(steps):
         r, v = orc.vv_step0(dt, r, v, f, s.mass)
-        r, f = chain_oracle_forces(orc, s, r)
+        r, f = chain_oracle_forces(orc, s, r, bonded=bonded)
         v = orc.vv_step1(dt, v, f, s.mass)
     ea = float(np.abs(min_image(ra - r, s.box)).max())
     eb = float(np.abs(min_image(rb - r, s.box)).max())
