@@ -764,7 +764,8 @@ __global__ void k_skin_offsets(Grid g, float4* __restrict__ out) {
 // pass does (the same FP32 vectors and the FP64 decision near the cutoff), compacted into
 // the neighbour-list rows the three-body kernels read (r_j - r_i with the site word, the cell
 // direction and rank), and the LJ 12-6 force of those pairs written as the site's force:
-// the first force term of the step (uniform sites, no clamp).  One wave per site.
+// the first force term of the step (uniform sites, no clamp).  One 16-lane DPP row per site,
+// four per wave.
 struct RefreshArgs {
     const SkinArgs* __restrict__ sk;
     const uint32_t* __restrict__ sorted;
@@ -783,82 +784,82 @@ struct RefreshArgs {
     double rc2;
 };
 
-__global__ __launch_bounds__(64 * kAtmWaves) void k_skin_refresh(RefreshArgs a) {
+constexpr int kRefWaves = 4;
+constexpr int kRefSites = 4 * kRefWaves;  // sites per workgroup: one 16-lane row each
+
+__global__ __launch_bounds__(64 * kRefWaves) void k_skin_refresh(RefreshArgs a) {
     __shared__ float4 woff[125];
-    __shared__ unsigned long long red_np[kAtmWaves];
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, sub = lane & 15;
     const int wvu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const SkinArgs* sk = a.sk;
     if (threadIdx.x < 125) woff[threadIdx.x] = sk->skin_off[threadIdx.x];
     __syncthreads();
     const int b = blockIdx.x;
-    const int p = ((b & 7) * a.chunk + (b >> 3)) * kAtmWaves + wvu;  // XCD-aware site slabs
+    // XCD-aware site slabs; the four rows of a wave take consecutive sites
+    const int p = (((b & 7) * a.chunk + (b >> 3)) * kRefWaves + wvu) * 4 + (lane >> 4);
+    const bool act = p < a.n;
+    const uint32_t self = act ? (uint32_t)p : 0u;  // dead rows read slot 0, class 0
+    const unsigned long long rmask = 0xffffull << (lane & 48);  // the lanes of my row
+    const float4* spos = sk->spos;
+    const uint32_t oi = act ? a.sorted[p] : 0u;
+    const float4 qi = spos[self];
+    const int nv = act ? (int)sk->vlc[p] : 0;
+    const uint32_t* vrow = sk->vl + (size_t)self * kSkinCap;
+    const uint16_t* drow = sk->vld + (size_t)self * kSkinCap;
+    float4* out = a.nl + (size_t)self * kFullCap;
+    uint16_t* dout = a.nld + (size_t)self * kFullCap;
+    double ljx = 0.0, ljy = 0.0, ljz = 0.0;
     unsigned long long npair = 0;
-    if (p < a.n) {
-        const uint32_t oi = a.sorted[p];
-        const float4* spos = sk->spos;
-        const float4 qi = spos[p];
-        const int nv = __builtin_amdgcn_readfirstlane((int)sk->vlc[p]);
-        const uint32_t* vrow = sk->vl + (size_t)p * kSkinCap;
-        const uint16_t* drow = sk->vld + (size_t)p * kSkinCap;
-        float4* out = a.nl + (size_t)p * kFullCap;
-        uint16_t* dout = a.nld + (size_t)p * kFullCap;
-        const uint32_t e0 = vrow[lane], e1 = vrow[64 + lane];
-        double ljx = 0.0, ljy = 0.0, ljz = 0.0;
-        int cnt = 0;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int s = h * 64 + lane;
-            const bool live = s < nv;
-            const uint32_t e = live ? (h ? e1 : e0) : (uint32_t)p;
-            const float4 q = spos[e & kSkinIdxMask];
-            const float4 o = woff[e >> kSkinIdxBits];
-            float4 d;
-            d.x = (q.x + o.x) - qi.x;
-            d.y = (q.y + o.y) - qi.y;
-            d.z = (q.z + o.z) - qi.z;
-            const uint32_t wj = __float_as_uint(q.w) | __float_as_uint(o.w);
-            d.w = __uint_as_float(wj);
-            const float r2 = live ? d.x * d.x + d.y * d.y + d.z * d.z : __builtin_huge_valf();
-            bool in = r2 <= a.rc2f;
-            const bool amb = fabsf(r2 - a.rc2f) <= a.band;
-            if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, oi, wj) > a.rc2);
-            const unsigned long long bal = ballot(in);
-            npair += (unsigned long long)__popcll(ballot(in && (wj & kIdxMask) > oi));
-            if (in) {
-                const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                if (slot < kFullCap) {
-                    out[slot] = d;
-                    dout[slot] = drow[s];
-                }
-                // LJ 12-6 of the pair (the pair pass's expression): force on i = -x fac
-                const float iX = __builtin_amdgcn_rcpf(r2);
-                const float s2 = a.sig2 * iX;
-                const float s6 = s2 * s2 * s2;
-                const float fac = a.eps24 * (2.f * s6 * s6 - s6) * iX;
-                ljx -= (double)(d.x * fac);
-                ljy -= (double)(d.y * fac);
-                ljz -= (double)(d.z * fac);
+    int cnt = 0;  // in-range entries of my row so far (the same in the row's lanes)
+    for (int t0 = 0; ballot(t0 < nv) != 0ull; t0 += 16) {  // wave-uniform trip count
+        const int s = t0 + sub;
+        const bool live = s < nv;
+        const uint32_t e = live ? vrow[s] : self;
+        const float4 q = spos[e & kSkinIdxMask];
+        const float4 o = woff[e >> kSkinIdxBits];
+        float4 d;
+        d.x = (q.x + o.x) - qi.x;
+        d.y = (q.y + o.y) - qi.y;
+        d.z = (q.z + o.z) - qi.z;
+        const uint32_t wj = __float_as_uint(q.w) | __float_as_uint(o.w);
+        d.w = __uint_as_float(wj);
+        const float r2 = live ? d.x * d.x + d.y * d.y + d.z * d.z : __builtin_huge_valf();
+        bool in = r2 <= a.rc2f;
+        const bool amb = fabsf(r2 - a.rc2f) <= a.band;
+        if (ballot(amb) != 0ull && amb) in = !(exact_r2(a.rx, a.ry, a.rz, a.g, oi, wj) > a.rc2);
+        const unsigned long long bal = ballot(in) & rmask;
+        npair += (unsigned long long)__popcll(ballot(in && (wj & kIdxMask) > oi));
+        if (in) {
+            // compacted in entry order, as the list pass orders its rows
+            const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (slot < kFullCap) {
+                out[slot] = d;
+                dout[slot] = drow[s];
             }
-            cnt += (int)__popcll(bal);
+            // LJ 12-6 of the pair (the pair pass's expression): force on i = -x fac
+            const float iX = __builtin_amdgcn_rcpf(r2);
+            const float s2 = a.sig2 * iX;
+            const float s6 = s2 * s2 * s2;
+            const float fac = a.eps24 * (2.f * s6 * s6 - s6) * iX;
+            ljx -= (double)(d.x * fac);
+            ljy -= (double)(d.y * fac);
+            ljz -= (double)(d.z * fac);
         }
-        const double fxs = wave_sum(ljx), fys = wave_sum(ljy), fzs = wave_sum(ljz);
-        if (lane == 0) {
-            a.nlc[p] = (uint32_t)min(cnt, kFullCap);
-            if (cnt > kFullCap) atomicOr(sk->flags, FLAG_NL_OVERFLOW);
-            a.fx[oi] = fxs;
-            a.fy[oi] = fys;
-            a.fz[oi] = fzs;
-        }
+        cnt += (int)__popcll(bal);
     }
-    if (lane == 0) red_np[wvu] = npair;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long tp = 0;
-        for (int k = 0; k < kAtmWaves; k++) tp += red_np[k];
-        atomicAdd(&sk->pair_counters[b], tp);
+    // the row's sum in every lane of the row
+    ljx += dpp_f64<0xB1>(ljx); ljx += dpp_f64<0x4E>(ljx); ljx += dpp_f64<0x124>(ljx); ljx += dpp_f64<0x128>(ljx);
+    ljy += dpp_f64<0xB1>(ljy); ljy += dpp_f64<0x4E>(ljy); ljy += dpp_f64<0x124>(ljy); ljy += dpp_f64<0x128>(ljy);
+    ljz += dpp_f64<0xB1>(ljz); ljz += dpp_f64<0x4E>(ljz); ljz += dpp_f64<0x124>(ljz); ljz += dpp_f64<0x128>(ljz);
+    if (act && sub == 0) {
+        a.nlc[p] = (uint32_t)min(cnt, kFullCap);
+        if (cnt > kFullCap) atomicOr(sk->flags, FLAG_NL_OVERFLOW);
+        a.fx[oi] = ljx;
+        a.fy[oi] = ljy;
+        a.fz[oi] = ljz;
     }
+    if (lane == 0 && npair) atomicAdd(&sk->pair_counters[b], npair);
 }
 
 // ---------------------------------------------------------------------------
@@ -1152,7 +1153,8 @@ hipError_t launch_skin_refresh(komd_ctx* c) {
     a.fx = d.fx; a.fy = d.fy; a.fz = d.fz;
     a.g = c->grid;
     a.n = (int)c->n;
-    const int blocks = atm_blocks(c);
+    // at most atm_blocks workgroups: the pair-count slots of the Verlet-list passes
+    const int blocks = 8 * ((((int)c->n + kRefSites - 1) / kRefSites + 7) / 8);
     a.chunk = blocks / 8;
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;
@@ -1160,7 +1162,7 @@ hipError_t launch_skin_refresh(komd_ctx* c) {
     a.sig2 = c->sig_u * c->sig_u;
     a.eps24 = 24.f * c->eps_u;
     ScopedTimer tm(c, "skin_refresh");
-    k_skin_refresh<<<blocks, 64 * kAtmWaves, 0, c->stream>>>(a);
+    k_skin_refresh<<<blocks, 64 * kRefWaves, 0, c->stream>>>(a);
     return hipGetLastError();
 }
 
