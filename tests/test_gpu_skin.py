@@ -264,11 +264,11 @@ def test_skin_lj_only_run(orc):
     assert dr < 1e-5
 
 
-def warmed_chains(nmol_xyz, seed):
+def warmed_chains(nmol_xyz, seed, sites=4):
     """Chains on a site grid of cells >= cutoff + skin after the capped warm-up of the
     benchmark (force cap raised from 20 to 2e5 in five 300-step stages at dt 0.001 with
     rescaling to T* 1.0), sites binned on their own positions."""
-    s = S.chains(nmol_xyz, 4, 0.7, seed=seed, rc=2.5 + SKIN)
+    s = S.chains(nmol_xyz, sites, 0.7, seed=seed, rc=2.5 + SKIN)
     topo = {"bonds": s.bonds, "angles": s.angles}
     for lim in (20.0, 200.0, 2000.0, 2e4, 2e5):
         with K.Context(s, cutoff=2.5, dt=0.001, limit_factor=lim, topology=topo, bin_mode=K.BIN_SITE) as c:
@@ -302,15 +302,15 @@ def chain_runs(s, topo, steps, bonded=True):
         return out, b.skin_rebuilds()
 
 
-@pytest.mark.parametrize("bonded,steps", [(True, 60), (False, 30)])
-def test_skin_chains_follow_fp64_run(orc, bonded, steps):
-    """2,048 sites of four-site chains, with their bonds and angles or without (LJ between
+@pytest.mark.parametrize("bonded,steps,sites", [(True, 60, 4), (False, 30, 4), (True, 30, 6), (False, 20, 6)])
+def test_skin_chains_follow_fp64_run(orc, bonded, steps, sites):
+    """512 chains of four or six sites, with their bonds and angles or without (LJ between
     molecules only): the same pair set every step as the per-step run (the lists leave
-    same-molecule pairs out; the per-site bonded kernel counts the in-range ones), forces at
-    the end equal to the oracle's, and both GPU runs track the FP64 trajectory of the oracle
-    (reference step order) equally well."""
-    s, topo = warmed_chains((8, 8, 8), 73)
-    print(f"skin chains n={s.n} bonded={bonded} steps={steps}", flush=True)
+    same-molecule pairs out; the bonded kernels count the in-range ones, six-site molecules
+    past their four-site register block), forces at the end equal to the oracle's, and both
+    GPU runs track the FP64 trajectory of the oracle (reference step order) equally well."""
+    s, topo = warmed_chains((8, 8, 8), 73, sites)
+    print(f"skin chains n={s.n} sites={sites} bonded={bonded} steps={steps}", flush=True)
     out, reb = chain_runs(s, topo, steps, bonded)
     (ra, va, _), ca = out[0]
     (rb, vb, fb), cb = out[1]
