@@ -784,22 +784,44 @@ struct RefreshArgs {
     double rc2;
 };
 
-constexpr int kRefWaves = 4;
-constexpr int kRefSites = 4 * kRefWaves;  // sites per workgroup: one 16-lane row each
+constexpr int kLjWaves = 4;
 
+// The sum over the L-lane group (L = 4, 8 or 16) in every lane of the group, a fixed order:
+// quads by quad_perm, then the two quads of a half-row by row_half_mirror (L = 8) or the
+// four quads of a row by row_ror (L = 16).
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+    v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+    if (L == 4) {
+    } else if (L == 8) {
+        v += dpp_f64<0x141>(v);  // row_half_mirror
+    } else {
+        v += dpp_f64<0x124>(v);  // row_ror:4
+        v += dpp_f64<0x128>(v);  // row_ror:8
+    }
+    return v;
+}
+
+constexpr int kRefWaves = 4;
+constexpr int kRefSites = 4 * kRefWaves;  // sites per workgroup
+
+// 16 lanes per site: narrower groups (the k_skin_lj rule) break the compacted row writes into
+// pieces of 64 B or less across many rows (256,000 atoms: 298 us at 4 lanes against 159).
 __global__ __launch_bounds__(64 * kRefWaves) void k_skin_refresh(RefreshArgs a) {
+    constexpr int L = 16, R = 64 / L;  // lanes per site, sites per wave
     __shared__ float4 woff[125];
-    const int lane = threadIdx.x & 63, sub = lane & 15;
+    const int lane = threadIdx.x & 63, sub = lane & (L - 1);
     const int wvu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const SkinArgs* sk = a.sk;
     if (threadIdx.x < 125) woff[threadIdx.x] = sk->skin_off[threadIdx.x];
     __syncthreads();
     const int b = blockIdx.x;
-    // XCD-aware site slabs; the four rows of a wave take consecutive sites
-    const int p = (((b & 7) * a.chunk + (b >> 3)) * kRefWaves + wvu) * 4 + (lane >> 4);
+    // XCD-aware site slabs; the R lane groups of a wave take consecutive sites
+    const int p = (((b & 7) * a.chunk + (b >> 3)) * kRefWaves + wvu) * R + lane / L;
     const bool act = p < a.n;
-    const uint32_t self = act ? (uint32_t)p : 0u;  // dead rows read slot 0, class 0
-    const unsigned long long rmask = 0xffffull << (lane & 48);  // the lanes of my row
+    const uint32_t self = act ? (uint32_t)p : 0u;  // dead groups read slot 0, class 0
+    const unsigned long long rmask = ((1ull << L) - 1ull) << (lane & (64 - L));  // my group's lanes
     const float4* spos = sk->spos;
     const uint32_t oi = act ? a.sorted[p] : 0u;
     const float4 qi = spos[self];
@@ -810,8 +832,8 @@ __global__ __launch_bounds__(64 * kRefWaves) void k_skin_refresh(RefreshArgs a) 
     uint16_t* dout = a.nld + (size_t)self * kFullCap;
     double ljx = 0.0, ljy = 0.0, ljz = 0.0;
     unsigned long long npair = 0;
-    int cnt = 0;  // in-range entries of my row so far (the same in the row's lanes)
-    for (int t0 = 0; ballot(t0 < nv) != 0ull; t0 += 16) {  // wave-uniform trip count
+    int cnt = 0;  // in-range entries of my group so far (the same in the group's lanes)
+    for (int t0 = 0; ballot(t0 < nv) != 0ull; t0 += L) {  // wave-uniform trip count
         const int s = t0 + sub;
         const bool live = s < nv;
         const uint32_t e = live ? vrow[s] : self;
@@ -848,10 +870,9 @@ __global__ __launch_bounds__(64 * kRefWaves) void k_skin_refresh(RefreshArgs a) 
         }
         cnt += (int)__popcll(bal);
     }
-    // the row's sum in every lane of the row
-    ljx += dpp_f64<0xB1>(ljx); ljx += dpp_f64<0x4E>(ljx); ljx += dpp_f64<0x124>(ljx); ljx += dpp_f64<0x128>(ljx);
-    ljy += dpp_f64<0xB1>(ljy); ljy += dpp_f64<0x4E>(ljy); ljy += dpp_f64<0x124>(ljy); ljy += dpp_f64<0x128>(ljy);
-    ljz += dpp_f64<0xB1>(ljz); ljz += dpp_f64<0x4E>(ljz); ljz += dpp_f64<0x124>(ljz); ljz += dpp_f64<0x128>(ljz);
+    ljx = group_sum<L>(ljx);
+    ljy = group_sum<L>(ljy);
+    ljz = group_sum<L>(ljz);
     if (act && sub == 0) {
         a.nlc[p] = (uint32_t)min(cnt, kFullCap);
         if (cnt > kFullCap) atomicOr(sk->flags, FLAG_NL_OVERFLOW);
@@ -869,8 +890,8 @@ __global__ __launch_bounds__(64 * kRefWaves) void k_skin_refresh(RefreshArgs a) 
 // added to the site's force (zero, or the step's bonded terms, which ran first: the order of
 // the per-step run), then Step1 and the next step's rebuild decision.  Lists of multi-site
 // molecules hold sites of other molecules only (the exclusion of LJ12_6.cpp:19 taken at the
-// build).  One 16-lane DPP row per site, four sites per wave; the row sums its FP64 partial
-// forces with quad_perm + row_ror, in a fixed order.
+// build).  A group of L lanes per site (skin_lanes), 64 / L sites per wave; the group sums
+// its FP64 partial forces by DPP (group_sum), in a fixed order.
 struct SkinLjArgs {
     const SkinArgs* __restrict__ sk;
     const uint32_t* __restrict__ sorted;
@@ -889,20 +910,20 @@ struct SkinLjArgs {
     float rc2f, band, sig2, eps24;
     double rc2, kick;
 };
-constexpr int kLjWaves = 4;
-constexpr int kLjSites = 4 * kLjWaves;  // sites per workgroup
-
+template <int L>
 __global__ __launch_bounds__(64 * kLjWaves) void k_skin_lj(SkinLjArgs a) {
+    static_assert(L == 4 || L == 8 || L == 16, "lanes per site");
+    constexpr int R = 64 / L;  // sites per wave
     __shared__ float4 woff[125];
     __shared__ unsigned long long red_np[kLjWaves];
-    const int lane = threadIdx.x & 63, sub = lane & 15;
+    const int lane = threadIdx.x & 63, sub = lane & (L - 1);
     const int wvu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const SkinArgs* sk = a.sk;
     if (threadIdx.x < 125) woff[threadIdx.x] = sk->skin_off[threadIdx.x];
     __syncthreads();
     const int b = blockIdx.x;
-    // XCD-aware site slabs; the four rows of a wave take consecutive sites
-    const int p = (((b & 7) * a.chunk + (b >> 3)) * kLjWaves + wvu) * 4 + (lane >> 4);
+    // XCD-aware site slabs; the R lane groups of a wave take consecutive sites
+    const int p = (((b & 7) * a.chunk + (b >> 3)) * kLjWaves + wvu) * R + lane / L;
     const bool act = p < a.n;
     const uint32_t self = act ? (uint32_t)p : 0u;  // dead lanes read slot 0, class 0
     const float4* spos = sk->spos;
@@ -912,7 +933,7 @@ __global__ __launch_bounds__(64 * kLjWaves) void k_skin_lj(SkinLjArgs a) {
     const uint32_t* vrow = sk->vl + (size_t)self * kSkinCap;
     double ljx = 0.0, ljy = 0.0, ljz = 0.0;
     unsigned long long npair = 0;
-    for (int t0 = 0; ballot(t0 < nv) != 0ull; t0 += 16) {  // wave-uniform trip count
+    for (int t0 = 0; ballot(t0 < nv) != 0ull; t0 += L) {  // wave-uniform trip count
         const int s = t0 + sub;
         const bool live = s < nv;
         const uint32_t e = live ? vrow[s] : self;
@@ -936,10 +957,9 @@ __global__ __launch_bounds__(64 * kLjWaves) void k_skin_lj(SkinLjArgs a) {
             ljz -= (double)(dz * fac);
         }
     }
-    // the row's sum in every lane of the row
-    ljx += dpp_f64<0xB1>(ljx); ljx += dpp_f64<0x4E>(ljx); ljx += dpp_f64<0x124>(ljx); ljx += dpp_f64<0x128>(ljx);
-    ljy += dpp_f64<0xB1>(ljy); ljy += dpp_f64<0x4E>(ljy); ljy += dpp_f64<0x124>(ljy); ljy += dpp_f64<0x128>(ljy);
-    ljz += dpp_f64<0xB1>(ljz); ljz += dpp_f64<0x4E>(ljz); ljz += dpp_f64<0x124>(ljz); ljz += dpp_f64<0x128>(ljz);
+    ljx = group_sum<L>(ljx);
+    ljy = group_sum<L>(ljy);
+    ljz = group_sum<L>(ljz);
     if (act && sub == 0) {
         const double nfx = (a.add ? a.fx[oi] : 0.0) + ljx, nfy = (a.add ? a.fy[oi] : 0.0) + ljy,
                      nfz = (a.add ? a.fz[oi] : 0.0) + ljz;
@@ -1115,6 +1135,17 @@ hipError_t launch_skin_list(komd_ctx* c, int par, bool dirs) {
     return hipGetLastError();
 }
 
+// Lanes per site of the Verlet-list LJ walk (k_skin_lj): the narrowest of 4, 8
+// and 16 that still gives about eight waves per SIMD (n L / 64 >= 8 x 1,024).  Narrow groups
+// share a wave's trip count and its per-site tail among more sites (262,144 chain sites:
+// 52.0 us at 4 lanes, 53.4 at 8, 68.8 at 16); small systems need the waves (4,000 atoms:
+// 0.0222 ms per step at 16 lanes, 0.0245 at 8).
+int skin_lanes(int64_t n) {
+    for (int L = 4; L < 16; L *= 2)
+        if (n * L >= 8 * 1024 * 64) return L;
+    return 16;
+}
+
 // LJ from the Verlet lists for a skinned step of parity par without a three-body term; add:
 // the step's bonded forces are already in f (else f is written outright).
 hipError_t launch_skin_lj(komd_ctx* c, int par, bool add) {
@@ -1130,7 +1161,9 @@ hipError_t launch_skin_lj(komd_ctx* c, int par, bool add) {
     a.add = add ? 1 : 0;
     a.par = par;
     // at most atm_blocks workgroups: the pair-count slots of the Verlet-list passes
-    const int blocks = 8 * ((((int)c->n + kLjSites - 1) / kLjSites + 7) / 8);
+    const int lanes = skin_lanes(c->n);
+    const int sites = kLjWaves * 64 / lanes;  // per workgroup
+    const int blocks = 8 * ((((int)c->n + sites - 1) / sites + 7) / 8);
     a.chunk = blocks / 8;
     a.rc2 = c->prm.cutoff * c->prm.cutoff;
     a.rc2f = (float)a.rc2;
@@ -1139,7 +1172,12 @@ hipError_t launch_skin_lj(komd_ctx* c, int par, bool add) {
     a.eps24 = 24.f * c->eps_u;
     a.kick = 0.5 * c->prm.delta_t;
     ScopedTimer tm(c, "skin_lj");
-    k_skin_lj<<<blocks, 64 * kLjWaves, 0, c->stream>>>(a);
+    if (lanes == 4)
+        k_skin_lj<4><<<blocks, 64 * kLjWaves, 0, c->stream>>>(a);
+    else if (lanes == 8)
+        k_skin_lj<8><<<blocks, 64 * kLjWaves, 0, c->stream>>>(a);
+    else
+        k_skin_lj<16><<<blocks, 64 * kLjWaves, 0, c->stream>>>(a);
     return hipGetLastError();
 }
 
