@@ -235,10 +235,12 @@ def test_set_sites_drops_a_topology_it_no_longer_fits():
 
 # ---- runs without a three-body term: LJ (and the bonded terms) from the Verlet lists
 
-def test_skin_lj_only_run(orc):
+@pytest.mark.parametrize("cells", [10, 30])
+def test_skin_lj_only_run(orc, cells):
     """Single-site LJ without a three-body term: the forces after the run equal the oracle's
-    LJ at the final positions, and every step has the pair set of a per-step run."""
-    s = argon(10, 72)
+    LJ at the final positions, and every step has the pair set of a per-step run.  4,000
+    atoms walk their lists with 16 lanes per site, 108,000 with 8."""
+    s = argon(cells, 72)
     print(f"skin LJ-only n={s.n}", flush=True)
     with K.Context(s) as a, K.Context(s) as b:
         b.set_skin(SKIN)
@@ -259,7 +261,7 @@ def test_skin_lj_only_run(orc):
     print(f"  rebuilds {reb}; forces vs oracle max {mx:.2e} rms {rms:.2e}; vs per-step |dr| {dr:.2e}; "
           f"counts {ca} {cb}")
     assert ca == cb
-    assert 1 <= reb < 20
+    assert 1 <= reb < 25
     assert mx < 2e-4 and rms < 5e-6
     assert dr < 1e-5
 
