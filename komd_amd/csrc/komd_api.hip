@@ -99,9 +99,29 @@ void drop_topology(komd_ctx* c) {
 void drop_graph(komd_ctx* c) {
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph_exec_odd) hipGraphExecDestroy(c->graph_exec_odd);
+    if (c->graph_exec_multi) hipGraphExecDestroy(c->graph_exec_multi);
     c->graph_exec = nullptr;
     c->graph_exec_odd = nullptr;
+    c->graph_exec_multi = nullptr;
     c->graph_key = -1;
+}
+
+// Steps per multi-step graph of komd_run: even (a skinned launch ends at the step parity it
+// started with) and a divisor of kFlagEvery.
+constexpr int kStepsPerGraph = 8;
+
+// Captures what fn queues on the context stream into an executable graph.
+template <class F>
+hipError_t capture_graph(komd_ctx* c, hipGraphExec_t* out, F fn) {
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t e1 = fn();
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (e1 == hipSuccess) e1 = e2;
+    if (e1 == hipSuccess) e1 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    return e1;
 }
 
 // Collect finished event pairs into the per-kernel totals.
@@ -859,6 +879,7 @@ namespace {
 // copy (kFlagEvery steps older) when it queues the next one, so an overflow or a non-finite
 // state stops the run within 2 kFlagEvery steps without draining the queue at every check.
 constexpr int64_t kFlagEvery = 32;
+static_assert(kFlagEvery % kStepsPerGraph == 0 && kStepsPerGraph % 2 == 0, "multi-step graphs");
 
 struct FlagWatch {
     bool pending[2] = {false, false};
@@ -883,8 +904,8 @@ struct FlagWatch {
 
 // komd_run with Verlet lists (skin_run): the first step always rebuilds; a step's last
 // kernel arms the next step's rebuild when some site would leave half the skin.  The step
-// parities alternate between two captured graphs (each reads its own rebuild word and
-// clears the other's).  The run ends with a container update without force reset, so the
+// parities alternate (each step reads its own rebuild word and clears the other's): two
+// single-step graphs, one per parity, and one graph of kStepsPerGraph steps from parity 0.  The run ends with a container update without force reset, so the
 // positions are wrapped into the box and the grid matches them, as after a plain run.
 komd_status run_skin(komd_ctx* c, int64_t nsteps, int kind3b, bool bonded) {
     HIPCALL(c, ensure_skin(c, kind3b != KOMD_3B_ATM_EXACT && kind3b != 0), "komd_run");
@@ -904,31 +925,39 @@ komd_status run_skin(komd_ctx* c, int64_t nsteps, int kind3b, bool bonded) {
         }
     } else {
         const int key = (1 << 20) + (bonded ? 1 << 16 : 0) + kind3b;
-        if (!c->graph_exec || !c->graph_exec_odd || c->graph_key != key) {
+        if (!c->graph_exec || !c->graph_exec_odd || !c->graph_exec_multi || c->graph_key != key) {
             drop_graph(c);
-            for (int par = 0; par < 2; par++) {
-                hipGraph_t graph = nullptr;
-                HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "komd_run capture");
-                hipError_t e = enqueue_skin_step(c, kind3b, par, bonded);
-                hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-                if (e == hipErrorNotSupported) {
-                    if (graph) hipGraphDestroy(graph);
-                    return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
-                }
-                HIPCALL(c, e, "komd_run capture");
-                HIPCALL(c, e2, "komd_run capture");
-                e = hipGraphInstantiate(par ? &c->graph_exec_odd : &c->graph_exec, graph, nullptr, nullptr, 0);
-                hipGraphDestroy(graph);
-                HIPCALL(c, e, "komd_run instantiate");
+            hipError_t e = hipSuccess;
+            for (int par = 0; par < 2 && e == hipSuccess; par++)
+                e = capture_graph(c, par ? &c->graph_exec_odd : &c->graph_exec,
+                                  [&] { return enqueue_skin_step(c, kind3b, par, bonded); });
+            if (e == hipSuccess)
+                e = capture_graph(c, &c->graph_exec_multi, [&] {
+                    hipError_t es = hipSuccess;
+                    for (int k = 0; k < kStepsPerGraph && es == hipSuccess; k++)
+                        es = enqueue_skin_step(c, kind3b, k & 1, bonded);
+                    return es;
+                });
+            if (e == hipErrorNotSupported) {
+                drop_graph(c);
+                return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
             }
+            if (e != hipSuccess) drop_graph(c);
+            HIPCALL(c, e, "komd_run capture");
             c->graph_key = key;
         }
         FlagWatch fw;
-        for (int64_t k = 0; k < nsteps; k++) {
-            HIPCALL(c, hipGraphLaunch(c->skin_par ? c->graph_exec_odd : c->graph_exec, c->stream), "komd_run launch");
-            c->skin_par ^= 1;
-            komd_status s = fw.after(c, k, nsteps);
-            if (s) return s;
+        for (int64_t k = 0; k < nsteps;) {
+            // whole multi-step graphs from parity 0, single steps to align and for the rest
+            const bool multi = c->skin_par == 0 && nsteps - k >= kStepsPerGraph;
+            const int steps = multi ? kStepsPerGraph : 1;
+            hipGraphExec_t ge = multi ? c->graph_exec_multi : (c->skin_par ? c->graph_exec_odd : c->graph_exec);
+            HIPCALL(c, hipGraphLaunch(ge, c->stream), "komd_run launch");
+            if (!multi) c->skin_par ^= 1;
+            for (int j = 0; j < steps; j++, k++) {
+                komd_status s = fw.after(c, k, nsteps);
+                if (s) return s;
+            }
         }
     }
     if (nsteps > 0) {
@@ -968,28 +997,32 @@ komd_status run_plain(komd_ctx* c, int64_t nsteps, int32_t kind3b, int32_t use_b
         }
     } else {
         const int key = (c->pair_kind * 8 + kind3b) * 4 + (bonded ? 2 : 0) + (thermo ? 1 : 0);
-        if (!c->graph_exec || c->graph_key != key) {
+        if (!c->graph_exec || !c->graph_exec_multi || c->graph_key != key) {
             drop_graph(c);
-            hipGraph_t graph = nullptr;
-            HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "komd_run capture");
-            hipError_t e = enqueue_step(c, kind3b, bonded, thermo);
-            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            hipError_t e = capture_graph(c, &c->graph_exec, [&] { return enqueue_step(c, kind3b, bonded, thermo); });
+            if (e == hipSuccess)
+                e = capture_graph(c, &c->graph_exec_multi, [&] {
+                    hipError_t es = hipSuccess;
+                    for (int k = 0; k < kStepsPerGraph && es == hipSuccess; k++)
+                        es = enqueue_step(c, kind3b, bonded, thermo);
+                    return es;
+                });
             if (e == hipErrorNotSupported) {
-                if (graph) hipGraphDestroy(graph);
+                drop_graph(c);
                 return fail(c, KOMD_ERR_STATE, "surrogate parameters not set");
             }
+            if (e != hipSuccess) drop_graph(c);
             HIPCALL(c, e, "komd_run capture");
-            HIPCALL(c, e2, "komd_run capture");
-            e = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
-            HIPCALL(c, e, "komd_run instantiate");
             c->graph_key = key;
         }
         FlagWatch fw;
-        for (int64_t k = 0; k < nsteps; k++) {
-            HIPCALL(c, hipGraphLaunch(c->graph_exec, c->stream), "komd_run launch");
-            s = fw.after(c, k, nsteps);
-            if (s) return s;
+        for (int64_t k = 0; k < nsteps;) {
+            const bool multi = nsteps - k >= kStepsPerGraph;
+            HIPCALL(c, hipGraphLaunch(multi ? c->graph_exec_multi : c->graph_exec, c->stream), "komd_run launch");
+            for (int j = 0; j < (multi ? kStepsPerGraph : 1); j++, k++) {
+                s = fw.after(c, k, nsteps);
+                if (s) return s;
+            }
         }
     }
     if (thermo && nsteps > 0) HIPCALL(c, launch_rescale(c, t0), "komd_run");

@@ -151,6 +151,9 @@ struct komd_ctx {
     // graph of one step (skinned runs: graph_exec for even steps, graph_exec_odd for odd ones)
     hipGraphExec_t graph_exec = nullptr;
     hipGraphExec_t graph_exec_odd = nullptr;
+    // kStepsPerGraph steps in one graph (skinned runs: starting at parity 0); the gap between
+    // two graph launches is several times that between two kernels of one graph
+    hipGraphExec_t graph_exec_multi = nullptr;
     int graph_key = -1;
     // Verlet lists (komd_set_skin): skin > 0 enables them for the runs that can use them;
     // skin_par is the parity of the next queued step

@@ -145,6 +145,25 @@ def test_skin_many_rebuild_segments_in_one_run(orc):
     assert mx < 2e-4 and rms < 5e-6
 
 
+@pytest.mark.parametrize("skin,kind", [(SKIN, K.ATM_EXACT), (0.0, K.ATM_EXACT), (SKIN, 0)])
+def test_graph_replay_matches_eager_steps(skin, kind):
+    """komd_run replays graphs of eight steps (from step parity 0 on Verlet lists) and of one
+    step; with kernel timing on it queues the same kernels step by step.  A run of 5 steps
+    and then one of 27 (which starts at the odd parity) must give bit-identical states."""
+    s = argon(10, 76)
+    out = []
+    for eager in (False, True):
+        with K.Context(s, nu=NU) as c:
+            c.set_skin(skin)
+            c.timing(eager)
+            c.compute_forces(kind)
+            c.run(5, kind3b=kind)
+            c.run(27, kind3b=kind)
+            out.append(c.get_state())
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
 def test_skin_list_overflow_is_an_error():
     """Density 1.1 with cutoff + skin = 3.1: about 136 sites within the list radius, over the
     list capacity of 128."""
