@@ -44,16 +44,16 @@ def argon(cells, seed, skin=SKIN, rho=0.8, T=0.8):
     return S.fcc(cells, rho, T, seed, rc=2.5 + skin)
 
 
-def oracle_forces(orc, s, r):
+def oracle_forces(orc, s, r, nthreads=8):
     t = s.copy()
     t.r = r
     rr, b = orc.update_container(t)
-    f_lj = orc.lj_c08(t, rr, b, nthreads=8)
-    f_atm = orc.atm_complete(t, rr, b, NU, coverage=0, sign=-1.0, nthreads=8)[0]
+    f_lj = orc.lj_c08(t, rr, b, nthreads=nthreads)
+    f_atm = orc.atm_complete(t, rr, b, NU, coverage=0, sign=-1.0, nthreads=nthreads)[0]
     return f_lj + f_atm
 
 
-@pytest.mark.parametrize("cells,seed,steps", [(10, 61, 6), (10, 62, 60), (20, 63, 40)])
+@pytest.mark.parametrize("cells,seed,steps", [(10, 61, 6), (10, 62, 60), (20, 63, 40), (40, 1004, 40)])
 def test_skin_forces_match_oracle(orc, cells, seed, steps):
     s = argon(cells, seed)
     print(f"skin forces n={s.n} steps={steps}", flush=True)
@@ -64,7 +64,7 @@ def test_skin_forces_match_oracle(orc, cells, seed, steps):
         reb = c.skin_rebuilds()
         r, _, f = c.get_state()
     assert np.all((r >= s.low) & (r <= s.high)), "the run must end with wrapped positions"
-    f_or = oracle_forces(orc, s, r)
+    f_or = oracle_forces(orc, s, r, nthreads=16 if s.n > 100000 else 8)
     mx, rms = rel_err(f, f_or)
     print(f"  rebuilds {reb}; forces vs oracle max {mx:.3e} rms {rms:.3e}")
     assert 1 <= reb < steps
