@@ -252,6 +252,54 @@ def test_set_sites_drops_a_topology_it_no_longer_fits():
         assert o["u_bond"] == 0 and o["u_angle"] == 0
 
 
+def trained_params():
+    from komd_amd.train import DEFAULT_WEIGHTS
+    return np.load(DEFAULT_WEIGHTS)
+
+
+@pytest.mark.parametrize("kind,steps,cells", [(K.SURROGATE, 60, 10), (K.HYBRID, 40, 10), (K.SURROGATE, 40, 40)])
+def test_skin_learned_run_matches_fresh_forces_and_plain_run(kind, steps, cells):
+    """The learned and hybrid kinds on Verlet lists: each step's rows come from the lists at
+    the current positions (k_skin_refresh), so the forces at the end of the run equal a fresh
+    force pass (container update, lists, LJ, three-body) at the same positions to the FP32
+    rounding of the list vectors, and the run follows the per-step run to the FP32 level of
+    the relative coordinates (the surrogate sums its descriptors in the list order)."""
+    s = argon(cells, 70)
+    p = trained_params()
+    center = float(s.low[0] + 0.5 * s.box[0])
+    print(f"skin learned kind={kind} n={s.n} steps={steps}", flush=True)
+    with K.Context(s, nu=NU) as a, K.Context(s, nu=NU) as b:
+        b.set_skin(SKIN)
+        out = []
+        for c in (a, b):
+            c.surrogate_set_params(p)
+            if kind == K.HYBRID:
+                c.set_hybrid(0, center, 2.0, 2.0)
+            c.compute_forces(kind)
+            c.run(steps, kind3b=kind)
+            out.append(c.get_state())
+        reb = b.skin_rebuilds()
+        rb, vb, fb = out[1]
+        b.compute_forces(kind)               # a fresh pass at the run's final positions
+        f_fresh = b.forces()
+    ra, va, fa = out[0]
+    mx, rms = rel_err(fb, f_fresh)
+    dr = float(np.abs(min_image(ra - rb, s.box)).max())
+    dv = float(np.abs(va - vb).max())
+    print(f"  rebuilds {reb}; forces vs fresh max {mx:.2e} rms {rms:.2e}; vs per-step run |dr| {dr:.2e} |dv| {dv:.2e}")
+    assert 1 <= reb < steps // 2
+    if s.n > 100000:
+        # The descriptors enter the MLP in BF16 (8-bit mantissa): an FP32-level difference of
+        # a descriptor between the two frames of reference flips its BF16 rounding at a few of
+        # the 256,000 sites, a step of 2^-8 in one input (256k: max 1.4e-4, RMS 1.6e-6; the
+        # model's own error against the exact ATM is 3e-2)
+        assert mx < 1e-3 and rms < 1e-5
+        assert dr < 1e-4 and dv < 1e-3
+    else:
+        assert mx < 1e-4 and rms < 1e-5
+        assert dr < 1e-5 and dv < 1e-4
+
+
 # ---- runs without a three-body term: LJ (and the bonded terms) from the Verlet lists
 
 @pytest.mark.parametrize("cells", [10, 30])
