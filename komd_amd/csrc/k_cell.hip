@@ -478,14 +478,15 @@ __global__ __launch_bounds__(64 * kAtmWaves) __attribute__((amdgpu_waves_per_eu(
             // r_j - r_i = (q_j + o) - q_i with q the FP32 positions relative to the build-time
             // cell centres and o the entry's cell offset (the list pass's staging arithmetic,
             // so a fresh list gives the same vectors); within a few ulps of rc the FP64
-            // distance decides, as in the list pass.  Both row halves are loaded at once
-            // (the row is allocated to its capacity); slots past the length read site p.
+            // distance decides, as in the list pass.  Both row halves are loaded at once, each
+            // lane only within the list's length; slots past the length read site p.
             const SkinArgs* sk = a.sk;
             const float4* spos = sk->spos;
             const float4 qi = spos[p];
             const int nv = __builtin_amdgcn_readfirstlane((int)sk->vlc[p]);
             const uint32_t* vrow = sk->vl + (size_t)p * kSkinCap;
-            const uint32_t e0 = vrow[lane], e1 = vrow[64 + lane];
+            const uint32_t e0 = lane < nv ? vrow[lane] : (uint32_t)p;
+            const uint32_t e1 = 64 + lane < nv ? vrow[64 + lane] : (uint32_t)p;
             int cnt = 0;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
